@@ -1,0 +1,253 @@
+"""Stream-collide throughput on one MI355X: D3Q19 256^3 lid-driven cube, MLUPs and
+fraction of the HBM roofline.
+
+Headline line (value): D3Q19 SRT, 256^3, FP32 populations, two-buffer pull
+streaming on the (x,y,z,q) layout with 8x8x8 tiles, bounce-back walls plus a
+moving lid u = 0.1 on +Y, Re = 100 (tau = 0.5 + 3 u N / Re), started from
+equilibrium at rho = 1, u = 0. K timed steps after W warm-up steps; inputs
+resident in HBM; timed with HIP events on the stream the kernels run on.
+
+The same line carries the Float16C + DDF-shift (double buffer and single-buffer
+esoteric pull) and FP32 esoteric-pull variants of the same case under "variants",
+the roofline of the dominant kernel, and the oracle (C++ restatement of the
+reference's per-node step, OpenMP over the host cores) timed on a bounded
+sample of the same workload as the CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only fp32]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(Q, nx, ny, nz, s, walls=True):
+    """Bytes per step: read + write every population of every fluid node, plus one flag byte per node."""
+    n_c = nx * ny * nz
+    n_f = (nx - 2) * (ny - 2) * (nz - 2) if walls else n_c
+    return n_f * 2 * Q * s + n_c
+
+
+def make_run(desc, u_lid):
+    import torch
+
+    from mojo_lbm_amd import _lib
+
+    L = _lib.lib()
+    h = _lib.Handle(desc)
+    fb, bb, gb = L.lbm_f_bytes(C.byref(desc)), L.lbm_bc_bytes(C.byref(desc)), L.lbm_flags_bytes(C.byref(desc))
+    f = torch.empty(fb, dtype=torch.uint8, device="cuda")
+    f2 = torch.empty(fb, dtype=torch.uint8, device="cuda")
+    bc = torch.zeros(bb, dtype=torch.uint8, device="cuda")
+    flags = torch.zeros(gb, dtype=torch.uint8, device="cuda")
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(L.lbm_init_equilibrium(h.ptr, f.data_ptr(), 1.0, None, st))
+    _lib.check(L.lbm_init_equilibrium(h.ptr, f2.data_ptr(), 1.0, None, st))
+    D = desc.D
+    lid = _lib.dvec([u_lid] + [0.0] * (D - 1))
+    zero = _lib.dvec([0.0] * D)
+    _lib.check(L.lbm_set_exterior_wall(h.ptr, flags.data_ptr(), bc.data_ptr(), b"+Y", 1, lid, 1.0, st))
+    for side in (b"-Y", b"+X", b"-X", b"-Z", b"+Z")[: (3 if D == 2 else 5)]:
+        _lib.check(L.lbm_set_exterior_wall(h.ptr, flags.data_ptr(), bc.data_ptr(), side, 1, zero, 1.0, st))
+    torch.cuda.synchronize()
+    return dict(h=h, f=f, f2=f2, bc=bc, flags=flags, L=L, st=st, desc=desc)
+
+
+def time_run(run, tau, steps, warmup):
+    """Runs warmup + steps launches; returns seconds for the timed steps (HIP events on the launch stream)."""
+    import torch
+
+    from mojo_lbm_amd import _lib
+
+    L, h, st, d = run["L"], run["h"], run["st"], run["desc"]
+    bufs = [run["f"], run["f2"]]
+    bc, flags = run["bc"].data_ptr(), run["flags"].data_ptr()
+    state = {"i": 0}
+
+    def one(k):
+        if d.method == 0:
+            a, b = bufs[k % 2], bufs[(k + 1) % 2]
+            return L.lbm_step_db(h.ptr, b.data_ptr(), a.data_ptr(), bc, flags, tau, st)
+        return L.lbm_step_ep(h.ptr, bufs[0].data_ptr(), bc, flags, tau, int(k % 2 == 0), st)
+
+    for k in range(warmup):
+        _lib.check(one(k))
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    k0 = warmup + (warmup % 2)  # keep the double-buffer parity aligned
+    for k in range(k0, k0 + steps):
+        rc = one(k)
+        if rc:
+            _lib.check(rc)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / 1e3
+
+
+def cpu_baseline(desc, tau, budget_s=12.0):
+    """Oracle (C++ restatement of the reference's per-node step, OpenMP) on the host cores:
+    256^3 lid-driven cube, as many full steps as fit in the budget (at least one)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    threads = O.max_threads()
+    O.set_threads(threads)
+    f, bc, flags = O.alloc(desc)
+    O.init_equilibrium(desc, f)
+    D = desc.D
+    O.set_exterior_wall(desc, flags, bc, "+Y", 1, [0.1] + [0.0] * (D - 1), 1.0)
+    for s in ["-Y", "+X", "-X", "-Z", "+Z"][: (3 if D == 2 else 5)]:
+        O.set_exterior_wall(desc, flags, bc, s, 1, [0.0] * D, 1.0)
+    f2 = f.copy()
+    t0 = time.perf_counter()
+    O.step_db(desc, f2, f, bc, flags, tau)
+    t1 = time.perf_counter()
+    per = t1 - t0
+    n_more = max(0, min(20, int(budget_s / per) - 1))
+    t0 = time.perf_counter()
+    for k in range(n_more):
+        if k % 2 == 0:
+            O.step_db(desc, f, f2, bc, flags, tau)
+        else:
+            O.step_db(desc, f2, f, bc, flags, tau)
+    t1 = time.perf_counter()
+    if n_more:
+        per = (t1 - t0) / n_more
+    n_c = desc.nx * desc.ny * desc.nz
+    del f, f2, bc, flags
+    return {
+        "value": round(n_c / per / 1e6, 2),
+        "unit": "MLUPs",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{desc.nx}^3 D3Q19 FP32 lid-driven cube, {1 + n_more} oracle steps (1 untimed), "
+                  f"OpenMP over {threads} threads",
+    }
+
+
+def load_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a profile summary
+    for this kernel and workload is committed under profiles/."""
+    p = ROOT / "profiles" / f"{name}.traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default=None, help="run one variant: fp32 | f16c | f16c_ep | fp32_ep")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (every box has one MI355X). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from mojo_lbm_amd._lib import LbmDesc
+
+    N = args.n
+    u_lid, Re = 0.1, 100.0
+    tau = 0.5 + 3.0 * u_lid * N / Re
+
+    def desc_for(storage, method, ddf):
+        return LbmDesc(D=3, Q=19, nx=N, ny=N, nz=N, tx=8, ty=8, tz=8, method=method, collision=0,
+                       storage=storage, ddf_shift=ddf, les=0, include_moving_boundary=1, included_bcs=3,
+                       smagorinsky_cs=0.1)
+
+    variants = {
+        "fp32": (0, 0, 0, 4),      # storage, method, ddf, bytes per population
+        "f16c": (1, 0, 1, 2),
+        "f16c_ep": (1, 1, 1, 2),
+        "fp32_ep": (0, 1, 0, 4),
+    }
+    names = [args.only] if args.only else list(variants)
+    results = {}
+    for name in names:
+        storage, method, ddf, s = variants[name]
+        d = desc_for(storage, method, ddf)
+        run = make_run(d, u_lid)
+        steps = args.steps if (name == "fp32" or args.only) else max(100, args.steps // 2)
+        t = time_run(run, tau, steps, args.warmup)
+        B = algorithmic_bytes(19, N, N, N, s)
+        ms = t / steps * 1e3
+        results[name] = {
+            "ms_per_step": ms,
+            "mlups": N ** 3 * steps / t / 1e6,
+            "gbs": B / (t / steps) / 1e9,
+            "bytes_per_step": B,
+        }
+        print(f"# {name}: {ms:.4f} ms/step, {results[name]['mlups']:.0f} MLUPs, "
+              f"{results[name]['gbs']:.0f} GB/s algorithmic ({results[name]['gbs'] / HBM_PEAK_GBS:.1%} of 8 TB/s)",
+              file=sys.stderr, flush=True)
+        del run
+        torch.cuda.empty_cache()
+
+    head = names[0]
+    r = results[head]
+    traffic = load_traffic(f"step_d3q19_{head}_{N}")
+    line = {
+        "metric": "D3Q19 256^3 MLUPs & % HBM roofline (fp32/fp16c); learned-LES vs Smagorinsky",
+        "value": round(r["mlups"], 1),
+        "unit": "MLUPs",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if variants[head][0] == 0 else "f16c-storage/f32-math",
+        "data": "synthetic (lid-driven cube initialised at equilibrium rho=1, u=0)",
+        "config": {
+            "workload": f"D3Q19 SRT lid-driven cube {N}^3, double-buffer pull, (x,y,z,q) 8x8x8 tiles, "
+                        f"u_lid=0.1, Re=100, tau={tau:.3f}",
+            "lattice": "D3Q19", "grid": [N, N, N], "tile": [8, 8, 8], "method": "double_buffer",
+            "storage": head, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(r["gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(r["gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 5),
+                         "roofline_frac": round(v["gbs"] / HBM_PEAK_GBS, 4)} for k, v in results.items()},
+    }
+    if "fp32" in results and "f16c" in results:
+        line["variants"]["f16c_over_fp32"] = round(results["f16c"]["mlups"] / results["fp32"]["mlups"], 3)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(desc_for(0, 0, 0), tau)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

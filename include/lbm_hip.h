@@ -1,0 +1,185 @@
+/*
+ * lbm_hip.h -- C ABI of the MI355X (gfx950) lattice Boltzmann stream-collide library.
+ *
+ * The library replaces the GPU side of mojo_lbm's solver step: the fused
+ * pull / esoteric-pull stream -> bounce-back / moving-wall / equilibrium BC ->
+ * SRT / KBC (+ Smagorinsky LES) collide -> store kernel, the rho/u output
+ * kernel and the host setup loops that fill f, flags and bc.
+ *
+ * Conventions
+ *  - Every buffer is caller-owned device memory, laid out exactly like the
+ *    reference's TileTensors: rank-4 (x,y,z,k) fields use the blocked product of
+ *    a column-major tile (tx,ty,tz,K) and a column-major tiler (Tx,Ty,Tz,1)
+ *    (src/lbm/layoutstruct.mojo:45-64):
+ *        addr(x,y,z,k) = lx + tx*(ly + ty*(lz + tz*k)) + tx*ty*tz*K*(bx + Tx*(by + Ty*bz))
+ *    with K = Q for f, K = D+1 for bc (u_0..u_{D-1}, rho) and K = 1 for flags.
+ *  - f is float (LBM_STORE_F32), uint16 Float16C codes (LBM_STORE_F16C) or
+ *    double (LBM_STORE_F64). bc, rho, u and tau are in the grid float type:
+ *    float for F32/F16C, double for F64.
+ *  - Calls are asynchronous on the given hipStream_t (passed as void*; NULL is
+ *    the default stream). Nothing is retained past a call except state owned by
+ *    the handle.
+ *  - Every function returns LBM_OK (0) or a negative status; lbm_last_error()
+ *    returns the calling thread's last message. No exception crosses the ABI.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Streaming methods: LBM_method, src/lbm/constants.mojo:20-31. */
+enum { LBM_DOUBLE_BUFFER = 0, LBM_ESOTERIC_PULL = 1 };
+
+/* Collision operators: Collisions, src/lbm/constants.mojo:41-55. */
+enum { LBM_COLL_SRT = 0, LBM_COLL_TRT = 1, LBM_COLL_KBC = 2, LBM_COLL_RLBM = 3 };
+
+/* Population storage: f32 (default), Float16C codes (src/utils/custom_fp.mojo:16-100),
+ * or f64 (the FP64 reference runs). */
+enum { LBM_STORE_F32 = 0, LBM_STORE_F16C = 1, LBM_STORE_F64 = 2 };
+
+/* Turbulence closure applied in collide (src/lbm/kernels/steps/collide.mojo:66-73). */
+enum { LBM_LES_NONE = 0, LBM_LES_SMAGORINSKY = 1 };
+
+/* Node flags: enum in the low 4 bits, bit masks above (src/lbm/constants.mojo:58-117). */
+enum {
+    LBM_FLAG_FLUID = 0,
+    LBM_FLAG_SOLID = 1,
+    LBM_FLAG_EQUILIBRIUM = 2,
+    LBM_FLAG_FLUID_SOLID_BOUNDARY = 3,
+    LBM_FLAG_CAPTURE_VELOCITY = 0x20,
+    LBM_FLAG_CAPTURE_DENSITY = 0x40,
+    LBM_FLAG_EXCLUDE = 0x80
+};
+
+/* Layout of the velocity output of lbm_macroscopic (both row-major). */
+enum { LBM_U_XYZD = 0, LBM_U_DXYZ = 1 };
+
+/* Status codes. */
+enum {
+    LBM_OK = 0,
+    LBM_ERR_INVALID = -1,     /* a validation error (bad desc, bad argument) */
+    LBM_ERR_UNSUPPORTED = -2, /* a valid configuration this build does not provide */
+    LBM_ERR_HIP = -3,         /* a HIP runtime error */
+    LBM_ERR_NO_DEVICE = -4    /* no usable gfx950 device */
+};
+
+/*
+ * Run description: the union of LBM_Grid's compile-time shape/tile parameters
+ * (src/lbm/grid.mojo:55-106) and LBM_Config's toggles (src/lbm/config.mojo:50-190).
+ */
+typedef struct lbm_desc {
+    int32_t D, Q;              /* (2,9) D2Q9, (3,19) D3Q19, (3,27) D3Q27 (src/lbm/lattice.mojo) */
+    int32_t nx, ny, nz;        /* lattice nodes; nz = 1 for D = 2 */
+    int32_t tx, ty, tz;        /* layout tile = launch block (grid.mojo:99, set_default_block_shape) */
+    int32_t method;            /* LBM_DOUBLE_BUFFER | LBM_ESOTERIC_PULL */
+    int32_t collision;         /* LBM_COLL_* */
+    int32_t storage;           /* LBM_STORE_* (Float16C: config.mojo use_float16c) */
+    int32_t ddf_shift;         /* store f - w_q (config.mojo DDF_shift) */
+    int32_t les;               /* LBM_LES_* (config.mojo LES) */
+    int32_t include_moving_boundary; /* config.mojo include_moving_boundary (reference default: 1) */
+    uint32_t included_bcs;     /* bit (1u << flag enum) per BC type in the run; FLUID and SOLID are implied
+                                  (config.mojo INCLUDED_BCs) */
+    double smagorinsky_cs;     /* Cs; the reference hard-codes 0.1 (steps/collide.mojo:71) */
+} lbm_desc;
+
+typedef struct lbm_handle lbm_handle;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int lbm_version(void);
+
+/* Last error message of the calling thread ("" when none). */
+const char* lbm_last_error(void);
+
+/* Number of visible HIP devices (0 when there is no GPU). */
+int lbm_device_count(int* count);
+
+/*
+ * Validates a description the way the reference's comptime asserts and host
+ * checks do (grid.mojo:419-440 divisibility, esoteric_pull.mojo:76-78 lattice
+ * pairing, config.mojo:128-150 BC set, KBC lattice restriction) and builds a
+ * handle bound to the current device. Replaces Solver.__init__
+ * (src/lbm/solver.mojo:63-76).
+ */
+int lbm_create(const lbm_desc* desc, lbm_handle** out);
+int lbm_destroy(lbm_handle* h);
+
+/* Byte sizes of the f, bc and flags buffers for a description. */
+size_t lbm_f_bytes(const lbm_desc* desc);
+size_t lbm_bc_bytes(const lbm_desc* desc);
+size_t lbm_flags_bytes(const lbm_desc* desc);
+
+/*
+ * One double-buffer step f_in -> f_out: replaces Solver.double_buffer_step
+ * (src/lbm/solver.mojo:81-111) and double_buffer_kernel
+ * (src/lbm/kernels/double_buffer.mojo:25-96). f_out and f_in must not alias.
+ */
+int lbm_step_db(lbm_handle* h, void* f_out, const void* f_in, const void* bc,
+                const uint8_t* flags, double tau, void* stream);
+
+/*
+ * One in-place esoteric-pull step: replaces Solver.esoteric_pull_step[is_even]
+ * (src/lbm/solver.mojo:115-154) and esoteric_pull_kernel
+ * (src/lbm/kernels/esoteric_pull.mojo:24-111). Solver.even_step/odd_step pass
+ * is_even = 1 / 0 (solver.mojo:159-187).
+ */
+int lbm_step_ep(lbm_handle* h, void* f, const void* bc, const uint8_t* flags,
+                double tau, int is_even, void* stream);
+
+/*
+ * rho (nx*ny*nz, row-major (x,y,z)) and u (row-major (x,y,z,d) or (d,x,y,z))
+ * after re-streaming and re-applying the BCs: replaces calculate_rho_and_velocity
+ * (src/lbm/output/velocity.mojo:25-125). after_odd selects the esoteric load
+ * pattern (ignored for double buffer).
+ */
+int lbm_macroscopic(lbm_handle* h, void* rho, void* u, const void* f, const void* bc,
+                    const uint8_t* flags, int after_odd, int u_layout, void* stream);
+
+/*
+ * FP64 totals over valid nodes (not SOLID, not EXCLUDE) of the same moments
+ * lbm_macroscopic reports: out_dev[0] = sum rho, out_dev[1+d] = sum rho*u_d.
+ * out_dev is device memory of D+1 doubles. Deterministic (fixed reduction
+ * order). No reference counterpart: the conservation check used to state mass
+ * drift.
+ */
+int lbm_reduce_moments_f64(lbm_handle* h, double* out_dev, const void* f, const void* bc,
+                           const uint8_t* flags, int after_odd, void* stream);
+
+/*
+ * Fill f with the equilibrium of (rho, u) at every node, stored with the
+ * method's pattern (double buffer: plain; esoteric pull: the odd store
+ * pattern): replaces initialize_fluid_at_rest / initialize_f_from_func without
+ * the gradient term (src/lbm/preprocess/initial_condition.mojo:47-178).
+ * u points to D host doubles (NULL = at rest).
+ */
+int lbm_init_equilibrium(lbm_handle* h, void* f, double rho, const double* u, void* stream);
+
+/*
+ * Write one exterior face: flags = flag and bc = (u, rho) at every node of the
+ * face, in the order the reference's host loop does (later calls overwrite
+ * shared edges): replaces set_exterior_walls
+ * (src/lbm/preprocess/boundary_condition.mojo:78-183), including _error_check
+ * (:20-70). side is one of "-X", "+X", "-Y", "+Y", "-Z", "+Z". u points to D
+ * host doubles or is NULL (free velocity, stored as NaN); rho NaN = free.
+ */
+int lbm_set_exterior_wall(lbm_handle* h, uint8_t* flags, void* bc, const char* side,
+                          int flag, const double* u, double rho, void* stream);
+
+/* Device memory helpers for hosts without their own GPU runtime (the Mojo host
+ * of ContextTileTensor, src/utils/contextTileTensor.mojo:113-158, 280-293). */
+int lbm_alloc(size_t bytes, void** ptr);
+int lbm_free(void* ptr);
+int lbm_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes, void* stream);
+int lbm_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes, void* stream);
+int lbm_memset(void* dst_dev, int value, size_t bytes, void* stream);
+int lbm_synchronize(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,299 @@
+// lbm_capi.hip -- the extern "C" boundary of liblbm_hip.so (declared in include/lbm_hip.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "lbm_launch.hpp"
+
+namespace lbm {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+static int hip_rc(hipError_t e, const char* what) {
+    if (e != hipSuccess) return set_error(LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+static bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+static int log2i(int v) {
+    int b = 0;
+    while ((1 << b) < v) ++b;
+    return b;
+}
+
+// Validation of a run description: the reference's comptime asserts and
+// host checks (grid.mojo:392-440, config.mojo:100-150, esoteric_pull.mojo:76-78,
+// KBC_.mojo:220) plus this library's own limits.
+static int validate(const lbm_desc* d) {
+    if (!d) return set_error(LBM_ERR_INVALID, "desc is NULL");
+    const bool lat_ok = (d->D == 2 && d->Q == 9) || (d->D == 3 && (d->Q == 19 || d->Q == 27));
+    if (!lat_ok) return set_error(LBM_ERR_INVALID, "lattice must be D2Q9, D3Q19 or D3Q27");
+    if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return set_error(LBM_ERR_INVALID, "grid shape must be positive");
+    if (d->D == 2 && (d->nz != 1 || d->tz != 1))
+        return set_error(LBM_ERR_INVALID, "a 2-D grid needs nz = 1 and tz = 1");
+    const int t[3] = {d->tx, d->ty, d->tz};
+    for (int i = 0; i < 3; ++i) {
+        if (!pow2(t[i])) return set_error(LBM_ERR_INVALID, "tile dimensions must be powers of two");
+        if (i < d->D && t[i] < 2) return set_error(LBM_ERR_INVALID, "tile dimensions must be > 1 along each lattice axis");
+    }
+    if (d->nx % d->tx || d->ny % d->ty || d->nz % d->tz)
+        return set_error(LBM_ERR_INVALID,
+                         "The grid shape along nx,ny and nz should be divisible by the block (tile) shape");
+    const long cells = (long)d->tx * d->ty * d->tz;
+    const long maxthreads = d->D == 2 ? 1024 : 512;
+    if (cells > maxthreads) return set_error(LBM_ERR_INVALID, "tile holds more nodes than one workgroup can (2-D: 1024, 3-D: 512)");
+    if (d->method != LBM_DOUBLE_BUFFER && d->method != LBM_ESOTERIC_PULL)
+        return set_error(LBM_ERR_INVALID, "method must be double buffer or esoteric pull");
+    if (d->collision == LBM_COLL_TRT || d->collision == LBM_COLL_RLBM)
+        return set_error(LBM_ERR_UNSUPPORTED, "TRT and RLBM collision are not provided by this build");
+    if (d->collision != LBM_COLL_SRT && d->collision != LBM_COLL_KBC)
+        return set_error(LBM_ERR_INVALID, "unknown collision operator");
+    if (d->storage < LBM_STORE_F32 || d->storage > LBM_STORE_F64) return set_error(LBM_ERR_INVALID, "unknown storage");
+    if (d->les != LBM_LES_NONE && d->les != LBM_LES_SMAGORINSKY) return set_error(LBM_ERR_INVALID, "unknown LES model");
+    if (!(d->smagorinsky_cs >= 0.0)) return set_error(LBM_ERR_INVALID, "smagorinsky_cs must be >= 0");
+    const double elems = (double)d->nx * d->ny * d->nz * d->Q;
+    if (elems >= 4294967296.0) return set_error(LBM_ERR_UNSUPPORTED, "more than 2^32 populations per buffer");
+    return LBM_OK;
+}
+
+static Geo make_geo(const lbm_desc* d) {
+    Geo g{};
+    g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
+    g.txb = log2i(d->tx); g.tyb = log2i(d->ty); g.tzb = log2i(d->tz);
+    g.Tx = d->nx / d->tx; g.Ty = d->ny / d->ty; g.Tz = d->nz / d->tz;
+    g.tile_cells = d->tx * d->ty * d->tz;
+    g.n_tiles = g.Tx * g.Ty * g.Tz;
+    g.xcd_chunk = (g.n_tiles % 8 == 0) ? g.n_tiles / 8 : 0;
+    return g;
+}
+
+// Dispatch on (Q, storage) to the matching Launch instantiation.
+template <class F> static int dispatch(const lbm_desc& d, F&& f) {
+#define LBM_CASE(QQ, SS, SID) \
+    if (d.Q == QQ && d.storage == SID) return f(Launch<Lattice<QQ>, SS>{});
+    LBM_CASE(9, StoreF32, LBM_STORE_F32)
+    LBM_CASE(9, StoreF16C, LBM_STORE_F16C)
+    LBM_CASE(9, StoreF64, LBM_STORE_F64)
+    LBM_CASE(19, StoreF32, LBM_STORE_F32)
+    LBM_CASE(19, StoreF16C, LBM_STORE_F16C)
+    LBM_CASE(19, StoreF64, LBM_STORE_F64)
+    LBM_CASE(27, StoreF32, LBM_STORE_F32)
+    LBM_CASE(27, StoreF16C, LBM_STORE_F16C)
+    LBM_CASE(27, StoreF64, LBM_STORE_F64)
+#undef LBM_CASE
+    return set_error(LBM_ERR_INVALID, "no kernel set for this lattice / storage");
+}
+
+static int check_handle(lbm_handle* h) {
+    if (!h) return set_error(LBM_ERR_INVALID, "handle is NULL");
+    return LBM_OK;
+}
+
+static Handle* H(lbm_handle* h) { return reinterpret_cast<Handle*>(h); }
+
+static size_t grid_bytes(const lbm_desc* d) { return d->storage == LBM_STORE_F64 ? 8 : 4; }
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+int lbm_version(void) { return 100; }
+
+const char* lbm_last_error(void) { return g_last_error.c_str(); }
+
+int lbm_device_count(int* count) {
+    if (!count) return set_error(LBM_ERR_INVALID, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return LBM_OK;
+}
+
+size_t lbm_f_bytes(const lbm_desc* d) {
+    if (validate(d)) return 0;
+    const size_t s = d->storage == LBM_STORE_F16C ? 2 : (d->storage == LBM_STORE_F64 ? 8 : 4);
+    return (size_t)d->nx * d->ny * d->nz * d->Q * s;
+}
+size_t lbm_bc_bytes(const lbm_desc* d) {
+    if (validate(d)) return 0;
+    return (size_t)d->nx * d->ny * d->nz * (d->D + 1) * grid_bytes(d);
+}
+size_t lbm_flags_bytes(const lbm_desc* d) {
+    if (validate(d)) return 0;
+    return (size_t)d->nx * d->ny * d->nz;
+}
+
+int lbm_create(const lbm_desc* desc, lbm_handle** out) {
+    if (!out) return set_error(LBM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate(desc);
+    if (rc) return rc;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(LBM_ERR_NO_DEVICE, "no HIP device visible");
+    Handle* h = new Handle();
+    h->d = *desc;
+    h->g = make_geo(desc);
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return set_error(LBM_ERR_NO_DEVICE, "hipGetDevice failed");
+    }
+    const int hz = desc->D == 3 ? desc->tz + 2 : 1;
+    h->lds_bytes = (size_t)(desc->tx + 2) * (desc->ty + 2) * hz;
+    const size_t pb = (size_t)h->g.n_tiles * (desc->D + 1) * sizeof(double);
+    if (hipMalloc((void**)&h->partial, pb) != hipSuccess) {
+        delete h;
+        return set_error(LBM_ERR_HIP, "hipMalloc of the reduction partials failed");
+    }
+    *out = reinterpret_cast<lbm_handle*>(h);
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_handle* hh) {
+    if (!hh) return LBM_OK;
+    Handle* h = H(hh);
+    if (h->partial) (void)hipFree(h->partial);
+    delete h;
+    return LBM_OK;
+}
+
+int lbm_step_db(lbm_handle* hh, void* f_out, const void* f_in, const void* bc, const uint8_t* flags, double tau,
+                void* stream) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (h.d.method != LBM_DOUBLE_BUFFER) return set_error(LBM_ERR_INVALID, "handle is not configured for double buffer");
+    if (!f_out || !f_in || !flags || !bc) return set_error(LBM_ERR_INVALID, "NULL buffer");
+    if (f_out == f_in) return set_error(LBM_ERR_INVALID, "f_out and f_in must be different buffers");
+    if (!(tau > 0.5)) return set_error(LBM_ERR_INVALID, "tau must be > 0.5");
+    return dispatch(h.d, [&](auto l) {
+        return decltype(l)::step(h, f_out, f_in, bc, flags, tau, MODE_DB, (hipStream_t)stream);
+    });
+}
+
+int lbm_step_ep(lbm_handle* hh, void* f, const void* bc, const uint8_t* flags, double tau, int is_even,
+                void* stream) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (h.d.method != LBM_ESOTERIC_PULL) return set_error(LBM_ERR_INVALID, "handle is not configured for esoteric pull");
+    if (!f || !flags || !bc) return set_error(LBM_ERR_INVALID, "NULL buffer");
+    if (!(tau > 0.5)) return set_error(LBM_ERR_INVALID, "tau must be > 0.5");
+    const int mode = is_even ? MODE_EP_EVEN : MODE_EP_ODD;
+    return dispatch(h.d, [&](auto l) {
+        return decltype(l)::step(h, f, f, bc, flags, tau, mode, (hipStream_t)stream);
+    });
+}
+
+static int macro_mode(const Handle& h, int after_odd) {
+    if (h.d.method == LBM_DOUBLE_BUFFER) return MODE_DB;
+    return after_odd ? MODE_EP_EVEN : MODE_EP_ODD;  // after an odd step the next step is even
+}
+
+int lbm_macroscopic(lbm_handle* hh, void* rho, void* u, const void* f, const void* bc, const uint8_t* flags,
+                    int after_odd, int u_layout, void* stream) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (!rho || !u || !f || !bc || !flags) return set_error(LBM_ERR_INVALID, "NULL buffer");
+    if (u_layout != LBM_U_XYZD && u_layout != LBM_U_DXYZ) return set_error(LBM_ERR_INVALID, "unknown u_layout");
+    const int mode = macro_mode(h, after_odd);
+    return dispatch(h.d, [&](auto l) {
+        return decltype(l)::macro(h, rho, u, f, bc, flags, mode, u_layout, (hipStream_t)stream);
+    });
+}
+
+int lbm_reduce_moments_f64(lbm_handle* hh, double* out_dev, const void* f, const void* bc, const uint8_t* flags,
+                           int after_odd, void* stream) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (!out_dev || !f || !bc || !flags) return set_error(LBM_ERR_INVALID, "NULL buffer");
+    const int mode = macro_mode(h, after_odd);
+    return dispatch(h.d, [&](auto l) {
+        return decltype(l)::reduce(h, out_dev, f, bc, flags, mode, (hipStream_t)stream);
+    });
+}
+
+int lbm_init_equilibrium(lbm_handle* hh, void* f, double rho, const double* u, void* stream) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (!f) return set_error(LBM_ERR_INVALID, "NULL buffer");
+    return dispatch(h.d, [&](auto l) { return decltype(l)::init(h, f, rho, u, (hipStream_t)stream); });
+}
+
+// _error_check, src/lbm/preprocess/boundary_condition.mojo:20-70.
+int lbm_set_exterior_wall(lbm_handle* hh, uint8_t* flags, void* bc, const char* side, int flag, const double* u,
+                          double rho, void* stream) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    const int D = h.d.D;
+    if (!flags || !bc) return set_error(LBM_ERR_INVALID, "NULL buffer");
+    const bool flag_ok = flag == LBM_FLAG_FLUID || flag == LBM_FLAG_SOLID ||
+                         (flag >= 0 && flag < 32 && ((h.d.included_bcs >> flag) & 1u));
+    if (!flag_ok) return set_error(LBM_ERR_INVALID, "Input Boundary Type is not in the config's valid boundary types");
+    if (!side || std::strlen(side) != 2 || (side[0] != '-' && side[0] != '+') ||
+        (side[1] != 'X' && side[1] != 'Y' && side[1] != 'Z'))
+        return set_error(LBM_ERR_INVALID, "Side not valid: expects one of -X, +X, -Y, +Y, -Z, +Z");
+    const int axis = side[1] - 'X';
+    if (axis >= D) return set_error(LBM_ERR_INVALID, "Side not valid for a 2-D grid");
+    const bool u_empty = u == nullptr;
+    if (u_empty && std::isnan(rho))
+        return set_error(LBM_ERR_INVALID, "Either velocity or density or both have to be specified. Both cant be left as None");
+    if (!u_empty)
+        for (int i = 0; i < D; ++i)
+            if (std::isnan(u[i]))
+                return set_error(LBM_ERR_INVALID, "either all components of velocity must be free or specified (cannot be NaN)");
+    if (flag == LBM_FLAG_SOLID) {
+        if (u_empty || std::isnan(rho)) return set_error(LBM_ERR_INVALID, "For Solid Type you must specify both u and rho");
+        if (!h.d.include_moving_boundary)
+            for (int i = 0; i < D; ++i)
+                if (u[i] != 0.0)
+                    return set_error(LBM_ERR_INVALID,
+                                     "include_moving_boundary in config was set to False but a non zero velocity was specified");
+    }
+    if (flag == LBM_FLAG_EQUILIBRIUM && !u_empty && std::isnan(rho))
+        return set_error(LBM_ERR_INVALID, "For Equilibrium BC, if u is specified then the density must also be specified");
+    const int n[3] = {h.d.nx, h.d.ny, h.d.nz};
+    const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    const int count = n[a1] * n[a2];
+    const unsigned blocks = (unsigned)((count + 255) / 256);
+    const double uu[3] = {u_empty ? NAN : u[0], u_empty ? NAN : u[1], (u_empty || D < 3) ? NAN : u[2]};
+    const int plus = side[0] == '+';
+    if (h.d.storage == LBM_STORE_F64)
+        hipLaunchKernelGGL((wall_kernel<double>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, flags, (double*)bc, h.g,
+                           D, axis, plus, (uint8_t)flag, uu[0], uu[1], uu[2], rho);
+    else
+        hipLaunchKernelGGL((wall_kernel<float>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, flags, (float*)bc, h.g, D,
+                           axis, plus, (uint8_t)flag, (float)uu[0], (float)uu[1], (float)uu[2], (float)rho);
+    return check_launch("wall_kernel");
+}
+
+int lbm_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return set_error(LBM_ERR_INVALID, "ptr is NULL");
+    return hip_rc(hipMalloc(ptr, bytes), "hipMalloc");
+}
+int lbm_free(void* ptr) { return hip_rc(hipFree(ptr), "hipFree"); }
+int lbm_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+    return hip_rc(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream), "hipMemcpyAsync");
+}
+int lbm_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+    return hip_rc(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream), "hipMemcpyAsync");
+}
+int lbm_memset(void* dst, int value, size_t bytes, void* stream) {
+    return hip_rc(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream), "hipMemsetAsync");
+}
+int lbm_synchronize(void* stream) {
+    return hip_rc(stream ? hipStreamSynchronize((hipStream_t)stream) : hipDeviceSynchronize(), "synchronize");
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// lbm_common.hpp -- lattice tables, Float16C codec, storage traits and launch
+// geometry shared by the gfx950 stream-collide kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/lbm_hip.h"
+
+namespace lbm {
+
+// ----------------------------------------------------------------- lattices
+// Velocity sets in the reference order with opposites adjacent (q odd, q+1):
+// get_D2Q9 src/lbm/lattice.mojo:302-351, get_D3Q19 :221-299, get_D3Q27 :122-218.
+// Tables are constexpr functions so that fully unrolled q loops fold them into
+// immediates.
+template <int Q_> struct Lattice;
+
+template <> struct Lattice<9> {
+    static constexpr int D = 2, Q = 9;
+    __host__ __device__ static constexpr int c(int q, int d) {
+        constexpr int8_t t[9][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},
+                                    {1, 1, 0},  {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0}};
+        return t[q][d];
+    }
+    __host__ __device__ static constexpr int wnum(int q) { return q == 0 ? 4 : 1; }
+    __host__ __device__ static constexpr int wden(int q) { return q == 0 ? 9 : (q < 5 ? 9 : 36); }
+};
+
+struct Cube27 {
+    __host__ __device__ static constexpr int c(int q, int d) {
+        constexpr int8_t t[27][3] = {
+            {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},    {0, -1, 0},  {0, 0, 1},  {0, 0, -1},
+            {1, 1, 0},   {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0},   {1, 0, 1},   {-1, 0, -1}, {1, 0, -1},
+            {-1, 0, 1},  {0, 1, 1},   {0, -1, -1}, {0, 1, -1},   {0, -1, 1},  {1, 1, 1},  {-1, -1, -1},
+            {1, 1, -1},  {-1, -1, 1}, {1, -1, 1},  {-1, 1, -1},  {-1, 1, 1},  {1, -1, -1}};
+        return t[q][d];
+    }
+    __host__ __device__ static constexpr int nnz(int q) {
+        return (c(q, 0) != 0) + (c(q, 1) != 0) + (c(q, 2) != 0);
+    }
+};
+
+template <> struct Lattice<19> {
+    static constexpr int D = 3, Q = 19;
+    __host__ __device__ static constexpr int c(int q, int d) { return Cube27::c(q, d); }
+    __host__ __device__ static constexpr int wnum(int) { return 1; }
+    __host__ __device__ static constexpr int wden(int q) {
+        return Cube27::nnz(q) == 0 ? 3 : (Cube27::nnz(q) == 1 ? 18 : 36);
+    }
+};
+
+template <> struct Lattice<27> {
+    static constexpr int D = 3, Q = 27;
+    __host__ __device__ static constexpr int c(int q, int d) { return Cube27::c(q, d); }
+    __host__ __device__ static constexpr int wnum(int q) {
+        return Cube27::nnz(q) == 0 ? 8 : (Cube27::nnz(q) == 1 ? 2 : 1);
+    }
+    __host__ __device__ static constexpr int wden(int q) {
+        return Cube27::nnz(q) == 0 ? 27 : (Cube27::nnz(q) == 1 ? 27 : (Cube27::nnz(q) == 2 ? 54 : 216));
+    }
+};
+
+// Opposite direction: the pair (q, q+1) for odd q (lattice.mojo:76-91 finds the same).
+__host__ __device__ constexpr int opp(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
+
+// Weight in the compute type, correctly rounded (the reference writes them as literals).
+template <class L, class R> __host__ __device__ __forceinline__ constexpr R weight(int q) {
+    return (R)L::wnum(q) / (R)L::wden(q);
+}
+
+// ----------------------------------------------------------------- Float16C
+// 1-4-11 format of Lehmann et al., PRE 106, 015308 (2022):
+//   e' = 0:  |v| = m * 2^-25        e' > 0: |v| = (2048 + m) * 2^(e' - 26)
+// Encoding rounds to nearest with ties away from zero and saturates to 0x7FFF
+// for magnitudes that round to 2 or more (and for inf / NaN).
+__device__ __forceinline__ float f16c_decode(uint32_t h) {
+    const uint32_t e = (h >> 11) & 0xFu;
+    const uint32_t m = h & 0x7FFu;
+    const uint32_t sig = e ? (m | 0x800u) : m;       // integer significand
+    const uint32_t k = e ? e : 1u;                   // grid exponent: 2^(k - 26)
+    const float scale = __uint_as_float((k + 101u) << 23);  // 2^(k-26) as an fp32 normal
+    const float mag = (float)sig * scale;            // exact (<= 12 significant bits)
+    return __uint_as_float(__float_as_uint(mag) | ((h & 0x8000u) << 16));
+}
+
+__device__ __forceinline__ uint32_t f16c_encode(float x) {
+    const uint32_t bits = __float_as_uint(x);
+    const uint32_t s = (bits >> 16) & 0x8000u;
+    const float a = __uint_as_float(bits & 0x7FFFFFFFu);
+    // Grid exponent E = max(floor(log2 a), -14); t = a * 2^(11 - E) lies in [2^11, 2^12)
+    // (or [0, 2^11) below the normal range); floor(t + 0.5) is exact round-half-up.
+    int E = (int)((bits >> 23) & 0xFFu) - 127;
+    E = E < -14 ? -14 : E;
+    const float t = fminf(a * __uint_as_float((uint32_t)(127 + 11 - E) << 23), 8192.0f);  // NaN -> 8192
+    const uint32_t r = (uint32_t)floorf(t + 0.5f);
+    uint32_t code = ((uint32_t)(E + 14) << 11) + r;
+    code = (a < 2.0f) ? code : 0x7FFFu;  // saturation; NaN compares false
+    code = code > 0x7FFFu ? 0x7FFFu : code;
+    return s | code;
+}
+
+// ----------------------------------------------------------------- storage
+// Populations are stored as S::T and computed in S::R (FP32 for F32 and F16C).
+struct StoreF32 {
+    using T = float;
+    using R = float;
+    static constexpr int id = LBM_STORE_F32;
+    __device__ __forceinline__ static R ld(const T* p) { return *p; }
+    __device__ __forceinline__ static void st(T* p, R v) { __builtin_nontemporal_store(v, p); }
+};
+struct StoreF16C {
+    using T = uint16_t;
+    using R = float;
+    static constexpr int id = LBM_STORE_F16C;
+    __device__ __forceinline__ static R ld(const T* p) { return f16c_decode(*p); }
+    __device__ __forceinline__ static void st(T* p, R v) {
+        __builtin_nontemporal_store((uint16_t)f16c_encode(v), p);
+    }
+};
+struct StoreF64 {
+    using T = double;
+    using R = double;
+    static constexpr int id = LBM_STORE_F64;
+    __device__ __forceinline__ static R ld(const T* p) { return *p; }
+    __device__ __forceinline__ static void st(T* p, R v) { __builtin_nontemporal_store(v, p); }
+};
+
+// ----------------------------------------------------------------- geometry
+// Launch geometry of one run. Tile dimensions are powers of two; one workgroup
+// processes one tile, one lane one node, lane id = local column-major index.
+struct Geo {
+    int nx, ny, nz;
+    int txb, tyb, tzb;    // log2 of the tile dimensions
+    int Tx, Ty, Tz;       // tiles per axis
+    int tile_cells;       // tx * ty * tz
+    int n_tiles;
+    int xcd_chunk;        // tiles per XCD when the XCD-contiguous remap is on, else 0
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X. With the
+// remap, XCD k walks a contiguous run of tiles, so the x- and y-neighbour tiles
+// whose halo cache lines a tile shares are resident in the same L2.
+__device__ __forceinline__ int tile_of_block(const Geo& g) {
+    const int b = blockIdx.x;
+    return g.xcd_chunk ? (b & 7) * g.xcd_chunk + (b >> 3) : b;
+}
+
+}  // namespace lbm

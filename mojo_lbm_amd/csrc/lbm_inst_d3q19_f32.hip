@@ -1,0 +1,7 @@
+// Kernel instantiations for one (velocity set, storage) pair; split per file so
+// the library builds in parallel.
+#include "lbm_launch.hpp"
+
+namespace lbm {
+template struct Launch<Lattice<19>, StoreF32>;
+}  // namespace lbm

@@ -1,0 +1,144 @@
+// lbm_launch.hpp -- host-side launchers: one Launch<Lattice, Storage> per
+// (velocity set, population storage); each member picks the kernel
+// instantiation for the run's method, collision, LES and DDF toggles.
+#pragma once
+
+#include <string>
+
+#include "lbm_kernels.hpp"
+
+namespace lbm {
+
+struct Handle {
+    lbm_desc d;
+    Geo g;
+    int device = 0;
+    size_t lds_bytes = 0;      // flag tile + halo
+    double* partial = nullptr; // per-tile reduction partials (n_tiles * (D+1))
+};
+
+int set_error(int code, const std::string& msg);
+int check_launch(const char* what);
+
+template <class L, class S> struct Launch {
+    using R = typename S::R;
+    using T = typename S::T;
+
+    template <int MODE, int COLL, bool LES, bool DDF>
+    static int step_k(const Handle& h, const StepParams<R>& p, hipStream_t st) {
+        hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
+                           h.lds_bytes, st, p);
+        return check_launch("step_kernel");
+    }
+
+    template <int MODE>
+    static int step_mode(const Handle& h, const StepParams<R>& p, hipStream_t st) {
+        const bool kbc = h.d.collision == LBM_COLL_KBC;
+        const bool les = h.d.les == LBM_LES_SMAGORINSKY;
+        const bool ddf = h.d.ddf_shift != 0;
+        if (!kbc && !les && !ddf) return step_k<MODE, C_SRT, false, false>(h, p, st);
+        if (!kbc && !les && ddf) return step_k<MODE, C_SRT, false, true>(h, p, st);
+        if (!kbc && les && !ddf) return step_k<MODE, C_SRT, true, false>(h, p, st);
+        if (!kbc && les && ddf) return step_k<MODE, C_SRT, true, true>(h, p, st);
+        if (kbc && !les && !ddf) return step_k<MODE, C_KBC, false, false>(h, p, st);
+        if (kbc && !les && ddf) return step_k<MODE, C_KBC, false, true>(h, p, st);
+        if (kbc && les && !ddf) return step_k<MODE, C_KBC, true, false>(h, p, st);
+        return step_k<MODE, C_KBC, true, true>(h, p, st);
+    }
+
+    static int step(const Handle& h, void* f_out, const void* f_in, const void* bc, const uint8_t* flags, double tau,
+                    int mode, hipStream_t st) {
+        StepParams<R> p;
+        p.f_in = f_in;
+        p.f_out = f_out;
+        p.bc = (const R*)bc;
+        p.flags = flags;
+        p.tau = (R)tau;
+        p.cs = (R)h.d.smagorinsky_cs;
+        p.moving = h.d.include_moving_boundary != 0;
+        p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
+        p.g = h.g;
+        if (mode == MODE_DB) return step_mode<MODE_DB>(h, p, st);
+        if (mode == MODE_EP_EVEN) return step_mode<MODE_EP_EVEN>(h, p, st);
+        return step_mode<MODE_EP_ODD>(h, p, st);
+    }
+
+    template <int MODE, bool DDF, bool REDUCE>
+    static int macro_k(const Handle& h, const MacroParams<R>& p, hipStream_t st) {
+        hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, REDUCE>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
+                           h.lds_bytes, st, p);
+        return check_launch("macro_kernel");
+    }
+
+    template <bool REDUCE>
+    static int macro_any(const Handle& h, const MacroParams<R>& p, int mode, hipStream_t st) {
+        const bool ddf = h.d.ddf_shift != 0;
+        if (mode == MODE_DB) return ddf ? macro_k<MODE_DB, true, REDUCE>(h, p, st) : macro_k<MODE_DB, false, REDUCE>(h, p, st);
+        if (mode == MODE_EP_EVEN)
+            return ddf ? macro_k<MODE_EP_EVEN, true, REDUCE>(h, p, st) : macro_k<MODE_EP_EVEN, false, REDUCE>(h, p, st);
+        return ddf ? macro_k<MODE_EP_ODD, true, REDUCE>(h, p, st) : macro_k<MODE_EP_ODD, false, REDUCE>(h, p, st);
+    }
+
+    static MacroParams<R> macro_params(const Handle& h, const void* f, const void* bc, const uint8_t* flags) {
+        MacroParams<R> p{};
+        p.f = f;
+        p.bc = (const R*)bc;
+        p.flags = flags;
+        p.moving = h.d.include_moving_boundary != 0;
+        p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
+        p.g = h.g;
+        return p;
+    }
+
+    static int macro(const Handle& h, void* rho, void* u, const void* f, const void* bc, const uint8_t* flags,
+                     int mode, int u_layout, hipStream_t st) {
+        MacroParams<R> p = macro_params(h, f, bc, flags);
+        p.rho = (R*)rho;
+        p.u = (R*)u;
+        p.u_layout = u_layout;
+        return macro_any<false>(h, p, mode, st);
+    }
+
+    static int reduce(const Handle& h, double* out, const void* f, const void* bc, const uint8_t* flags, int mode,
+                      hipStream_t st) {
+        MacroParams<R> p = macro_params(h, f, bc, flags);
+        p.partial = h.partial;
+        int rc = macro_any<true>(h, p, mode, st);
+        if (rc) return rc;
+        if (L::D == 2) hipLaunchKernelGGL((reduce_finish<3>), dim3(1), dim3(256), 0, st, h.partial, h.g.n_tiles, out);
+        else hipLaunchKernelGGL((reduce_finish<4>), dim3(1), dim3(256), 0, st, h.partial, h.g.n_tiles, out);
+        return check_launch("reduce_finish");
+    }
+
+    static int init(const Handle& h, void* f, double rho_d, const double* u_d, hipStream_t st) {
+        QVals<R> v{};
+        R u[3] = {0, 0, 0};
+        if (u_d)
+            for (int d = 0; d < L::D; ++d) u[d] = (R)u_d[d];
+        const R rho = (R)rho_d;
+        const R uu = dot_uu<L, R>(u);
+        const bool ddf = h.d.ddf_shift != 0;
+        for (int q = 0; q < L::Q; ++q) v.v[q] = ddf ? feq<L, R, true>(q, rho, u, uu) : feq<L, R, false>(q, rho, u, uu);
+        const size_t n = (size_t)h.g.n_tiles * h.g.tile_cells * L::Q;
+        const unsigned blocks = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL((fill_kernel<S>), dim3(blocks), dim3(256), 0, st, (T*)f, v, L::Q, (uint32_t)h.g.tile_cells, n);
+        return check_launch("fill_kernel");
+    }
+};
+
+// Launchers are instantiated once per (lattice, storage) in lbm_inst_*.hip.
+#define LBM_FOR_EACH_INSTANCE(X)        \
+    X(Lattice<9>, StoreF32)             \
+    X(Lattice<9>, StoreF16C)            \
+    X(Lattice<9>, StoreF64)             \
+    X(Lattice<19>, StoreF32)            \
+    X(Lattice<19>, StoreF16C)           \
+    X(Lattice<19>, StoreF64)            \
+    X(Lattice<27>, StoreF32)            \
+    X(Lattice<27>, StoreF16C)           \
+    X(Lattice<27>, StoreF64)
+
+#define LBM_EXTERN_INSTANCE(L, S) extern template struct Launch<L, S>;
+LBM_FOR_EACH_INSTANCE(LBM_EXTERN_INSTANCE)
+
+}  // namespace lbm

@@ -1,0 +1,713 @@
+/*
+ * lbm_oracle.cpp -- CPU restatement of mojo_lbm's stream-collide path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP library: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The
+ * product path never calls it.
+ *
+ * It restates, per node and in the reference's operation order, the functions
+ * of the reference's double-buffer and esoteric-pull kernels. The reference is
+ * Mojo GPU code (no CPU stepping path); its per-cell building blocks are written
+ * CPU/GPU-generic (src/lbm/kernels/ReadMe.md:5,11,19), which is what this file
+ * follows. Arithmetic is templated on the compute type T (float or double);
+ * populations are stored as float, double or Float16C codes. Build with
+ * -ffp-contract=off so every operation is rounded as written.
+ *
+ * Pinning (what each function is checked against, in tests/):
+ *   lattice tables ........ moment identities (sum w = 1, sum w c = 0, sum w c c = I/3), opposite pairs
+ *   f16c_encode/decode .... exhaustive 65,536-code table against the format definition
+ *                           (Lehmann et al., PRE 106, 015308 (2022)), round-to-nearest brute force
+ *   addr (tiled layout) ... bijection + explicit formula
+ *   pull streaming ........ a periodic delta moves one node per step per direction
+ *   halfway bounce-back ... a population reflected by a wall returns in one step
+ *   moving wall ........... momentum injection closed form; Couette profile
+ *   SRT / f_eq ............ f_eq moments reproduce rho and rho u; f = f_eq is a fixed point
+ *   DDF shift ............. shifted and unshifted FP64 runs agree to rounding
+ *   esoteric pull ......... bitwise equality with double buffer in periodic boxes; slot ownership
+ *   Smagorinsky ........... closed form on a synthetic shear f_neq
+ *   KBC ................... gamma = 2 reduces to SRT; conservation; D2Q9 shear table
+ *   equilibrium BC ........ closed form (node reset to f_eq of prescribed rho, u)
+ *   LDC D2Q9 Re=100 ....... Ghia et al. centreline fixture (benchmarks/LDC/*), loose physics check
+ */
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/lbm_hip.h"
+
+namespace {
+
+/* ---------------------------------------------------------------- lattices
+ * Velocity sets in reference order, opposites adjacent (q odd, q+1):
+ * get_D2Q9 src/lbm/lattice.mojo:302-351, get_D3Q19 :221-299, get_D3Q27 :122-218.
+ */
+struct Lat {
+    int D, Q;
+    int c[27][3];
+    int wnum[27], wden[27];  // weight = wnum / wden, evaluated in T
+    int opp[27];
+};
+
+const int C27[27][3] = {
+    {0, 0, 0},
+    {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+    {1, 1, 0}, {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0},
+    {1, 0, 1}, {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1},
+    {0, 1, 1}, {0, -1, -1}, {0, 1, -1}, {0, -1, 1},
+    {1, 1, 1}, {-1, -1, -1}, {1, 1, -1}, {-1, -1, 1},
+    {1, -1, 1}, {-1, 1, -1}, {-1, 1, 1}, {1, -1, -1}};
+
+const int C9[9][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},
+                      {1, 1, 0},  {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0}};
+
+Lat make_lat(int D, int Q) {
+    Lat L{};
+    L.D = D;
+    L.Q = Q;
+    for (int q = 0; q < Q; ++q) {
+        for (int d = 0; d < 3; ++d) L.c[q][d] = (Q == 9) ? C9[q][d] : C27[q][d];
+        int nnz = std::abs(L.c[q][0]) + std::abs(L.c[q][1]) + std::abs(L.c[q][2]);
+        if (Q == 9) {
+            const int num[3] = {4, 1, 1}, den[3] = {9, 9, 36};
+            L.wnum[q] = num[nnz]; L.wden[q] = den[nnz];
+        } else if (Q == 19) {
+            const int num[3] = {1, 1, 1}, den[3] = {3, 18, 36};
+            L.wnum[q] = num[nnz]; L.wden[q] = den[nnz];
+        } else {
+            const int num[4] = {8, 2, 1, 1}, den[4] = {27, 27, 54, 216};
+            L.wnum[q] = num[nnz]; L.wden[q] = den[nnz];
+        }
+    }
+    // _get_opposite_indices, src/lbm/lattice.mojo:76-91: search for -c.
+    for (int q = 0; q < Q; ++q)
+        for (int k = 0; k < Q; ++k)
+            if (L.c[k][0] == -L.c[q][0] && L.c[k][1] == -L.c[q][1] && L.c[k][2] == -L.c[q][2]) {
+                L.opp[q] = k;
+                break;
+            }
+    return L;
+}
+
+/* ---------------------------------------------------------------- Float16C
+ * 1-4-11 format of Lehmann et al. (2022) (cited at src/utils/custom_fp.mojo:24-31):
+ * code = s | e' << 11 | m, value = m * 2^-25 for e' = 0, (2048 + m) * 2^(e'-26) otherwise.
+ * Encoding rounds to the nearest code with ties away from zero and saturates to
+ * 0x7FFF for magnitudes that round to 2 or more, infinities and NaN
+ * (custom_fp.mojo:71-100 behaviour). Written here from that definition.
+ */
+float f16c_decode(uint16_t h) {
+    int e = (h >> 11) & 0xF;
+    int m = h & 0x7FF;
+    double mag = (e == 0) ? std::ldexp((double)m, -25) : std::ldexp((double)(2048 + m), e - 26);
+    float v = (float)mag;  // exact: at most 12 significant bits
+    return (h & 0x8000) ? -v : v;
+}
+
+uint16_t f16c_encode(float x) {
+    uint32_t bits;
+    std::memcpy(&bits, &x, 4);
+    uint16_t s = (uint16_t)((bits >> 16) & 0x8000);
+    double a = std::fabs((double)x);
+    if (!(a < 2.0)) return s | 0x7FFF;  // >= 2, inf, NaN
+    // Exponent of the code grid containing a: E = max(floor(log2 a), -14).
+    int E = -14;
+    if (a >= std::ldexp(1.0, -14)) {
+        int ex;
+        std::frexp(a, &ex);  // a = f * 2^ex, f in [0.5, 1)
+        E = ex - 1;
+    }
+    double t = std::ldexp(a, 11 - E);      // exact
+    double r = std::floor(t + 0.5);        // round half away from zero (t >= 0)
+    long code = ((long)(E + 14) << 11) + (long)r;
+    if (code > 0x7FFF) code = 0x7FFF;
+    return s | (uint16_t)code;
+}
+
+/* ---------------------------------------------------------------- layout
+ * Blocked product of col-major tile (tx,ty,tz,K) and col-major tiler (Tx,Ty,Tz,1):
+ * src/lbm/layoutstruct.mojo:45-64, inverse idx_to_ijk src/lbm/geometry/interpolated_BB.mojo:24-43.
+ */
+struct Geo {
+    int nx, ny, nz, tx, ty, tz, Tx, Ty, Tz;
+    size_t tile_cells;
+    size_t addr(int x, int y, int z, int k, int K) const {
+        size_t lx = x % tx, ly = y % ty, lz = z % tz;
+        size_t bx = x / tx, by = y / ty, bz = z / tz;
+        return lx + tx * (ly + ty * (lz + tz * (size_t)k)) + tile_cells * K * (bx + Tx * (by + Ty * bz));
+    }
+};
+
+Geo make_geo(const lbm_desc* d) {
+    Geo g;
+    g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
+    g.tx = d->tx; g.ty = d->ty; g.tz = d->tz;
+    g.Tx = d->nx / d->tx; g.Ty = d->ny / d->ty; g.Tz = d->nz / d->tz;
+    g.tile_cells = (size_t)d->tx * d->ty * d->tz;
+    return g;
+}
+
+/* get_adjacent_idx, src/lbm/kernels/utils/index.mojo:44-64: floor-mod wrap, axes >= D stay 0. */
+inline void adjacent(const Geo& g, int D, const int idx[3], const int c[3], int shift, int out[3]) {
+    const int n[3] = {g.nx, g.ny, g.nz};
+    for (int d = 0; d < 3; ++d) {
+        if (d < D) {
+            int v = (idx[d] + shift * c[d]) % n[d];
+            if (v < 0) v += n[d];
+            out[d] = v;
+        } else {
+            out[d] = 0;
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- storage */
+template <class S, class T> struct Store;
+template <class T> struct Store<float, T> {
+    static T load(const float* p, size_t i) { return (T)p[i]; }
+    static void store(float* p, size_t i, T v) { p[i] = (float)v; }
+};
+template <class T> struct Store<double, T> {
+    static T load(const double* p, size_t i) { return (T)p[i]; }
+    static void store(double* p, size_t i, T v) { p[i] = (double)v; }
+};
+/* load_f / store_f with Float16C, src/lbm/kernels/utils/load_and_store.mojo:17-110:
+ * the stored value is encoded from its fp32 value. */
+template <class T> struct Store<uint16_t, T> {
+    static T load(const uint16_t* p, size_t i) { return (T)f16c_decode(p[i]); }
+    static void store(uint16_t* p, size_t i, T v) { p[i] = f16c_encode((float)v); }
+};
+
+inline bool flag_is_solid(uint8_t f) { return (f & 0x0F) == LBM_FLAG_SOLID; }     // Flags.is[SOLID]
+inline bool flag_is_eq(uint8_t f) { return (f & 0x0F) == LBM_FLAG_EQUILIBRIUM; }  // Flags.is[EQUILIBRIUM]
+inline bool flag_excluded(uint8_t f) { return (f & LBM_FLAG_EXCLUDE) != 0; }     // Flags.has[EXCLUDE]
+
+/* ---------------------------------------------------------------- per-cell math
+ * All in compute type T, in the order the reference writes them.
+ */
+template <class T> struct Cell {
+    const Lat& L;
+    T w[27];
+    explicit Cell(const Lat& lat) : L(lat) {
+        for (int q = 0; q < L.Q; ++q) w[q] = (T)L.wnum[q] / (T)L.wden[q];
+    }
+
+    /* Vector.dot, src/utils/vector.mojo:205-217: out = 0; out += a_i * b_i in order. */
+    T dot_c(const T* u, int q) const {
+        T out = 0;
+        for (int d = 0; d < L.D; ++d) out += u[d] * (T)L.c[q][d];
+        return out;
+    }
+    T dot_uu(const T* u) const {
+        T out = 0;
+        for (int d = 0; d < L.D; ++d) out += u[d] * u[d];
+        return out;
+    }
+
+    /* f_eq, src/lbm/kernels/utils/equilibrium.mojo:10-38. */
+    T feq(int q, T rho, const T* u, T uu, bool ddf) const {
+        T eu = dot_c(u, q);
+        if (ddf) return w[q] * rho * ((T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * uu) + w[q] * (rho - (T)1);
+        return w[q] * rho * ((T)1 + (T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * uu);
+    }
+
+    /* get_density, src/lbm/kernels/utils/moment.mojo:17-41 (Vector.sum, vector.mojo:219-224). */
+    T density(const T* f, bool ddf) const {
+        T s = 0;
+        for (int q = 0; q < L.Q; ++q) s += f[q];
+        return ddf ? s + (T)1 : s;
+    }
+    /* get_velocity, src/lbm/kernels/utils/moment.mojo:45-76. */
+    void velocity(const T* f, T rho, T* u) const {
+        for (int d = 0; d < L.D; ++d) u[d] = 0;
+        for (int q = 0; q < L.Q; ++q)
+            for (int d = 0; d < L.D; ++d) u[d] += f[q] * (T)L.c[q][d];
+        for (int d = 0; d < L.D; ++d) u[d] /= rho;
+    }
+
+    int nstress() const { return L.D * (L.D + 1) / 2; }
+    /* get_stress_indices, src/lbm/lattice.mojo:354-398. */
+    void stress_pair(int n, int& a, int& b) const {
+        static const int p2[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+        static const int p3[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+        if (L.D == 2) { a = p2[n][0]; b = p2[n][1]; }
+        else { a = p3[n][0]; b = p3[n][1]; }
+    }
+    /* get_non_eq_second_order_moment + get_Qiab, src/lbm/kernels/utils/moment.mojo:80-150. */
+    void pi_neq(const T* fneq, T* pi) const {
+        for (int n = 0; n < nstress(); ++n) {
+            int a, b;
+            stress_pair(n, a, b);
+            T s = 0;
+            for (int q = 0; q < L.Q; ++q) s += fneq[q] * (T)L.c[q][a] * (T)L.c[q][b];
+            pi[n] = s;
+        }
+    }
+
+    /* Smagorinsky tau_eddy: get_strain_rate_tensor (moment.mojo:154-176),
+     * get_strain_rate_tensor_norm_squared (moment.mojo:182-215),
+     * get_Smagorinsky_LES_tau (src/lbm/kernels/ops/turbulence.mojo:12-38).
+     * The strain rate uses the base tau (steps/collide.mojo:69). */
+    T smagorinsky_tau(const T* pi, T rho, T tau, T Cs) const {
+        const T cs2 = (T)1 / (T)3;
+        T k = (T)(-1) / ((T)2 * rho * cs2 * tau);
+        T snorm = 0;
+        for (int n = 0; n < nstress(); ++n) {
+            int a, b;
+            stress_pair(n, a, b);
+            T s = pi[n] * k;
+            T ss = s * s;
+            snorm += (a != b) ? ss * (T)2 : ss;
+        }
+        T v_eddy = (Cs * Cs) * std::sqrt((T)2 * snorm);
+        return (T)3 * v_eddy;
+    }
+
+    /* KBC shear part. D2Q9: get_shear_D2Q9, src/lbm/kernels/ops/collisions/KBC_.mojo:137-175.
+     * D3Q19/D3Q27 (no reference code): the same construction, the second-order Hermite
+     * projection of the deviatoric non-equilibrium stress,
+     *   s_q = w_q / (2 cs^4) * sum_ab (c_a c_b - cs^2 delta_ab) * dev(Pi)_ab,
+     * which reproduces the D2Q9 table term by term. */
+    void kbc_shear(const T* pi, T* s) const {
+        if (L.D == 2) {
+            T N = pi[0] - pi[2];
+            s[0] = 0;
+            s[1] = N * (T)0.25;
+            s[2] = N * (T)0.25;
+            s[3] = -N * (T)0.25;
+            s[4] = -N * (T)0.25;
+            s[5] = pi[1] * (T)0.25;
+            s[6] = pi[1] * (T)0.25;
+            s[7] = -pi[1] * (T)0.25;
+            s[8] = -pi[1] * (T)0.25;
+            return;
+        }
+        // dev(Pi) for D = 3: Pi - tr(Pi)/3 I. H:dev = sum_ab c_a c_b dev_ab (the -cs^2 tr(dev)
+        // term vanishes): diagonal terms where c_a != 0, off-diagonal terms (xy, xz, yz) with the
+        // sign of c_a c_b, each summed in that order; w / (2 cs^4) = 4.5 w.
+        T tr3 = (pi[0] + pi[3] + pi[5]) / (T)3;
+        const T dev[3] = {pi[0] - tr3, pi[3] - tr3, pi[5] - tr3};
+        const T off[3] = {pi[1], pi[2], pi[4]};
+        const int pa[3] = {0, 0, 1}, pb[3] = {1, 2, 2};
+        for (int q = 0; q < L.Q; ++q) {
+            T diag = 0, offs = 0;
+            for (int dd = 0; dd < 3; ++dd)
+                if (L.c[q][dd] != 0) diag = diag + dev[dd];
+            for (int n = 0; n < 3; ++n) {
+                int cc = L.c[q][pa[n]] * L.c[q][pb[n]];
+                if (cc != 0) offs = offs + (cc > 0 ? off[n] : -off[n]);
+            }
+            s[q] = w[q] * (T)4.5 * (diag + (T)2 * offs);
+        }
+    }
+};
+
+/* ---------------------------------------------------------------- one node */
+template <class S, class T> struct Stepper {
+    const lbm_desc& d;
+    Lat L;
+    Geo g;
+    Cell<T> cell;
+    T Cs;
+    Stepper(const lbm_desc& desc) : d(desc), L(make_lat(desc.D, desc.Q)), g(make_geo(&desc)), cell(L) {
+        Cs = (T)desc.smagorinsky_cs;
+    }
+    int K() const { return L.Q; }
+    int Kbc() const { return L.D + 1; }
+
+    /* set_adjacent_flags, src/lbm/kernels/ops/load_and_store.mojo:270-291 (pull neighbours x - c). */
+    void gather_flags(const uint8_t* flags, const int idx[3], uint8_t pf[27]) const {
+        pf[0] = flags[g.addr(idx[0], idx[1], idx[2], 0, 1)];
+        for (int q = 1; q < L.Q; ++q) {
+            int n[3];
+            adjacent(g, L.D, idx, L.c[q], -1, n);
+            pf[q] = flags[g.addr(n[0], n[1], n[2], 0, 1)];
+        }
+    }
+
+    /* double_buffer_pull_load_f_vec, src/lbm/kernels/ops/load_and_store.mojo:164-221:
+     * raw-flag compare against SOLID selects halfway bounce-back (:216). */
+    void load_db(const S* f, const uint8_t pf[27], const int idx[3], T* fv) const {
+        for (int q = 0; q < L.Q; ++q) {
+            int n[3];
+            adjacent(g, L.D, idx, L.c[q], -1, n);
+            if (pf[q] == LBM_FLAG_SOLID)
+                fv[q] = Store<S, T>::load(f, g.addr(idx[0], idx[1], idx[2], L.opp[q], K()));
+            else
+                fv[q] = Store<S, T>::load(f, g.addr(n[0], n[1], n[2], q, K()));
+        }
+    }
+
+    /* esoteric_pull_load_f_vec, src/lbm/kernels/ops/load_and_store.mojo:11-86. */
+    void load_ep(const S* f, const int idx[3], bool even, T* fv) const {
+        fv[0] = Store<S, T>::load(f, g.addr(idx[0], idx[1], idx[2], 0, K()));
+        for (int p = 1; p < L.Q - 1; p += 2) {
+            int nq = p + 1;
+            int s[3];
+            if (even) {
+                adjacent(g, L.D, idx, L.c[nq], -1, s);  // pull index of the negative direction
+                fv[p] = Store<S, T>::load(f, g.addr(idx[0], idx[1], idx[2], p, K()));
+                fv[nq] = Store<S, T>::load(f, g.addr(s[0], s[1], s[2], nq, K()));
+            } else {
+                adjacent(g, L.D, idx, L.c[p], 1, s);  // push index of the positive direction
+                fv[p] = Store<S, T>::load(f, g.addr(idx[0], idx[1], idx[2], nq, K()));
+                fv[nq] = Store<S, T>::load(f, g.addr(s[0], s[1], s[2], p, K()));
+            }
+        }
+    }
+
+    /* esoteric_pull_store_f_vec, src/lbm/kernels/ops/load_and_store.mojo:89-159. */
+    void store_ep(S* f, const int idx[3], bool even, const T* fv) const {
+        Store<S, T>::store(f, g.addr(idx[0], idx[1], idx[2], 0, K()), fv[0]);
+        for (int nq = 2; nq < L.Q; nq += 2) {
+            int p = nq - 1;
+            int s[3];
+            if (even) {
+                adjacent(g, L.D, idx, L.c[nq], -1, s);
+                Store<S, T>::store(f, g.addr(s[0], s[1], s[2], nq, K()), fv[p]);
+                Store<S, T>::store(f, g.addr(idx[0], idx[1], idx[2], p, K()), fv[nq]);
+            } else {
+                adjacent(g, L.D, idx, L.c[p], 1, s);
+                Store<S, T>::store(f, g.addr(s[0], s[1], s[2], p, K()), fv[p]);
+                Store<S, T>::store(f, g.addr(idx[0], idx[1], idx[2], nq, K()), fv[nq]);
+            }
+        }
+    }
+
+    /* moving_wall_bc, src/lbm/kernels/ops/boundary_condition.mojo:17-83:
+     * f_q += 2*3*w_q*rho_w*(c_q . u_w) for q >= 1 whose pull neighbour is SOLID (masked enum). */
+    void moving_wall(T* fv, const uint8_t pf[27], const T* bc, const int idx[3]) const {
+        for (int q = 1; q < L.Q; ++q) {
+            if (!flag_is_solid(pf[q])) continue;
+            int n[3];
+            adjacent(g, L.D, idx, L.c[q], -1, n);
+            T u[3];
+            for (int dd = 0; dd < L.D; ++dd) u[dd] = bc[g.addr(n[0], n[1], n[2], dd, Kbc())];
+            T rho = bc[g.addr(n[0], n[1], n[2], L.D, Kbc())];
+            fv[q] += (T)6 * cell.w[q] * rho * cell.dot_c(u, q);
+        }
+    }
+
+    /* equilibrium_bc + get_density_and_velocity_for_eq_BC,
+     * src/lbm/kernels/ops/boundary_condition.mojo:87-204: out-of-bounds pulls count as the rest value. */
+    void equilibrium(T* fv, const uint8_t pf[27], const T* bc, const int idx[3]) const {
+        if (!flag_is_eq(pf[0])) return;
+        bool ddf = d.ddf_shift != 0;
+        T ub[3];
+        for (int dd = 0; dd < L.D; ++dd) ub[dd] = bc[g.addr(idx[0], idx[1], idx[2], dd, Kbc())];
+        T rb = bc[g.addr(idx[0], idx[1], idx[2], L.D, Kbc())];
+        const int n[3] = {g.nx, g.ny, g.nz};
+        T vel[3] = {0, 0, 0};
+        T rho = 0;
+        for (int q = 0; q < L.Q; ++q) {
+            T rest = ddf ? (T)0 : cell.w[q];
+            bool oob = false;
+            for (int i = 0; i < 3; ++i) {
+                if (i < L.D) {
+                    int pi = idx[i] - L.c[q][i];
+                    oob = (pi < 0 || pi >= n[i]) || oob;
+                }
+            }
+            T fq = oob ? rest : fv[q];
+            rho += fq;
+            for (int dd = 0; dd < L.D; ++dd) vel[dd] += fq * (T)L.c[q][dd];
+        }
+        if (ddf) rho += (T)1;
+        for (int dd = 0; dd < L.D; ++dd) vel[dd] /= rho;
+        const T* ul = std::isnan(ub[0]) ? vel : ub;
+        T rl = std::isnan(rb) ? rho : rb;
+        T uu = cell.dot_uu(ul);
+        T feq[27];
+        for (int q = 0; q < L.Q; ++q) feq[q] = cell.feq(q, rl, ul, uu, ddf);
+        for (int q = 0; q < L.Q; ++q) fv[q] = feq[q];
+    }
+
+    /* collide, src/lbm/kernels/steps/collide.mojo:23-90 (SRT collision.mojo:12-46, KBC KBC_.mojo:180-238). */
+    void collide(T* fv, T tau) const {
+        bool ddf = d.ddf_shift != 0;
+        T rho = cell.density(fv, ddf);
+        T u[3];
+        cell.velocity(fv, rho, u);
+        T tau_local = tau;
+        bool need_neq = d.les == LBM_LES_SMAGORINSKY || d.collision == LBM_COLL_KBC;
+        T fneq[27], pi[6];
+        if (need_neq) {
+            T uu = cell.dot_uu(u);
+            for (int q = 0; q < L.Q; ++q) fneq[q] = fv[q] - cell.feq(q, rho, u, uu, ddf);
+            cell.pi_neq(fneq, pi);
+            if (d.les == LBM_LES_SMAGORINSKY) tau_local += cell.smagorinsky_tau(pi, rho, tau, Cs);
+        }
+        if (d.collision == LBM_COLL_KBC) {
+            T beta = (T)1 / ((T)2 * tau_local);
+            T inv_beta = (T)2 * tau_local;
+            T uu = cell.dot_uu(u);
+            T feq[27], ds[27], dh[27];
+            for (int q = 0; q < L.Q; ++q) feq[q] = cell.feq(q, rho, u, uu, false);
+            cell.kbc_shear(pi, ds);
+            for (int q = 0; q < L.Q; ++q) dh[q] = fneq[q] - ds[q];
+            T sp1 = 0, sp2 = 0;
+            for (int q = 0; q < L.Q; ++q) sp1 += ds[q] * dh[q] / feq[q];
+            for (int q = 0; q < L.Q; ++q) sp2 += dh[q] * dh[q] / feq[q];
+            T gamma = inv_beta - ((T)2 - inv_beta) * sp1 / (sp2 + (T)1e-32);
+            for (int q = 0; q < L.Q; ++q) fv[q] -= beta * (ds[q] * (T)2 + gamma * dh[q]);
+            return;
+        }
+        // SRT
+        T uu = cell.dot_uu(u);
+        T inv_tau = (T)1 / tau_local;
+        for (int q = 0; q < L.Q; ++q) fv[q] -= inv_tau * (fv[q] - cell.feq(q, rho, u, uu, ddf));
+    }
+
+    bool valid(uint8_t flag) const { return !(flag_is_solid(flag) || flag_excluded(flag)); }  // is_valid_thread
+    bool eq_enabled() const { return (d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u; }
+
+    /* stream + apply_boundary_conditions for one node; returns false for invalid nodes. */
+    void gather(const S* f, const T* bc, const uint8_t* flags, const int idx[3], int mode, T* fv,
+                uint8_t pf[27]) const {
+        // mode: 0 double buffer, 1 esoteric even, 2 esoteric odd
+        if (mode == 0) {
+            gather_flags(flags, idx, pf);
+            load_db(f, pf, idx, fv);
+        } else {
+            load_ep(f, idx, mode == 1, fv);
+            pf[0] = flags[g.addr(idx[0], idx[1], idx[2], 0, 1)];
+            if (d.include_moving_boundary) gather_flags(flags, idx, pf);
+        }
+        if (d.include_moving_boundary) moving_wall(fv, pf, bc, idx);
+        if (eq_enabled()) equilibrium(fv, pf, bc, idx);
+    }
+};
+
+int g_threads = 1;
+
+template <class S, class T>
+int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uint8_t* flags, double tau,
+              int mode) {
+    Stepper<S, T> st(*d);
+    const Geo& g = st.g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const T tau_t = (T)tau;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (long i = 0; i < ncell; ++i) {
+        int idx[3] = {(int)(i % g.nx), (int)((i / g.nx) % g.ny), (int)(i / ((long)g.nx * g.ny))};
+        uint8_t flag = flags[g.addr(idx[0], idx[1], idx[2], 0, 1)];
+        if (!st.valid(flag)) continue;
+        T fv[27];
+        uint8_t pf[27];
+        st.gather(f_in, bc, flags, idx, mode, fv, pf);
+        st.collide(fv, tau_t);
+        if (mode == 0) {
+            for (int q = 0; q < st.L.Q; ++q) Store<S, T>::store(f_out, g.addr(idx[0], idx[1], idx[2], q, st.K()), fv[q]);
+        } else {
+            st.store_ep(f_out, idx, mode == 1, fv);
+        }
+    }
+    return 0;
+}
+
+/* calculate_rho_and_velocity, src/lbm/output/velocity.mojo:25-125 (tau = 0, solid nodes report bc). */
+template <class S, class T>
+int macro_impl(const lbm_desc* d, T* rho_out, T* u_out, const S* f, const T* bc, const uint8_t* flags, int mode,
+               int u_layout) {
+    Stepper<S, T> st(*d);
+    const Geo& g = st.g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const int D = st.L.D;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (long i = 0; i < ncell; ++i) {
+        int idx[3] = {(int)(i % g.nx), (int)((i / g.nx) % g.ny), (int)(i / ((long)g.nx * g.ny))};
+        uint8_t flag = flags[g.addr(idx[0], idx[1], idx[2], 0, 1)];
+        T rho, u[3] = {0, 0, 0};
+        if (flag != LBM_FLAG_SOLID) {
+            T fv[27];
+            uint8_t pf[27];
+            st.gather(f, bc, flags, idx, mode, fv, pf);
+            rho = st.cell.density(fv, d->ddf_shift != 0);
+            st.cell.velocity(fv, rho, u);
+        } else {
+            for (int dd = 0; dd < D; ++dd) u[dd] = bc[g.addr(idx[0], idx[1], idx[2], dd, D + 1)];
+            rho = bc[g.addr(idx[0], idx[1], idx[2], D, D + 1)];
+        }
+        size_t r = ((size_t)idx[0] * g.ny + idx[1]) * g.nz + idx[2];
+        rho_out[r] = rho;
+        for (int dd = 0; dd < D; ++dd) {
+            if (u_layout == LBM_U_XYZD) u_out[r * D + dd] = u[dd];
+            else u_out[(size_t)dd * ncell + r] = u[dd];
+        }
+    }
+    return 0;
+}
+
+/* FP64 sums of rho and rho*u over valid nodes, in node order (x fastest). */
+template <class S, class T>
+int reduce_impl(const lbm_desc* d, double* out, const S* f, const T* bc, const uint8_t* flags, int mode) {
+    Stepper<S, T> st(*d);
+    const Geo& g = st.g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const int D = st.L.D;
+    std::vector<double> part((size_t)ncell * (D + 1), 0.0);
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (long i = 0; i < ncell; ++i) {
+        int idx[3] = {(int)(i % g.nx), (int)((i / g.nx) % g.ny), (int)(i / ((long)g.nx * g.ny))};
+        uint8_t flag = flags[g.addr(idx[0], idx[1], idx[2], 0, 1)];
+        if (!st.valid(flag)) continue;
+        T fv[27];
+        uint8_t pf[27];
+        st.gather(f, bc, flags, idx, mode, fv, pf);
+        T rho = st.cell.density(fv, d->ddf_shift != 0);
+        T u[3];
+        st.cell.velocity(fv, rho, u);
+        part[(size_t)i * (D + 1)] = (double)rho;
+        for (int dd = 0; dd < D; ++dd) part[(size_t)i * (D + 1) + 1 + dd] = (double)rho * (double)u[dd];
+    }
+    for (int k = 0; k <= D; ++k) out[k] = 0.0;
+    for (long i = 0; i < ncell; ++i)
+        for (int k = 0; k <= D; ++k) out[k] += part[(size_t)i * (D + 1) + k];
+    return 0;
+}
+
+/* initialize_f_from_func with a uniform velocity and no gradient term,
+ * src/lbm/preprocess/initial_condition.mojo:86-178 (esoteric: odd store pattern, :172-174). */
+template <class S, class T>
+int init_impl(const lbm_desc* d, S* f, double rho_d, const double* u_d) {
+    Stepper<S, T> st(*d);
+    const Geo& g = st.g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    T rho = (T)rho_d, u[3] = {0, 0, 0};
+    if (u_d)
+        for (int dd = 0; dd < st.L.D; ++dd) u[dd] = (T)u_d[dd];
+    T uu = st.cell.dot_uu(u);
+    T fv[27];
+    for (int q = 0; q < st.L.Q; ++q) fv[q] = st.cell.feq(q, rho, u, uu, d->ddf_shift != 0);
+    for (long i = 0; i < ncell; ++i) {
+        int idx[3] = {(int)(i % g.nx), (int)((i / g.nx) % g.ny), (int)(i / ((long)g.nx * g.ny))};
+        if (d->method == LBM_ESOTERIC_PULL) st.store_ep(f, idx, false, fv);
+        else
+            for (int q = 0; q < st.L.Q; ++q) Store<S, T>::store(f, g.addr(idx[0], idx[1], idx[2], q, st.K()), fv[q]);
+    }
+    return 0;
+}
+
+int check(const lbm_desc* d) {
+    if (!d) return -1;
+    bool lat_ok = (d->D == 2 && d->Q == 9) || (d->D == 3 && (d->Q == 19 || d->Q == 27));
+    if (!lat_ok) return -1;
+    if (d->tx <= 0 || d->ty <= 0 || d->tz <= 0) return -1;
+    if (d->nx % d->tx || d->ny % d->ty || d->nz % d->tz) return -1;
+    if (d->D == 2 && (d->nz != 1 || d->tz != 1)) return -1;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+uint16_t orc_f16c_encode(float x) { return f16c_encode(x); }
+float orc_f16c_decode(uint16_t h) { return f16c_decode(h); }
+void orc_f16c_encode_n(const float* x, uint16_t* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = f16c_encode(x[i]);
+}
+void orc_f16c_decode_n(const uint16_t* h, float* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = f16c_decode(h[i]);
+}
+
+/* Lattice tables for tests: c (Q x 3 ints), weights (double, num/den), opposite indices. */
+int orc_lattice(int D, int Q, int* c_out, double* w_out, int* opp_out) {
+    lbm_desc d{};
+    d.D = D; d.Q = Q;
+    bool ok = (D == 2 && Q == 9) || (D == 3 && (Q == 19 || Q == 27));
+    if (!ok) return -1;
+    Lat L = make_lat(D, Q);
+    for (int q = 0; q < Q; ++q) {
+        for (int k = 0; k < 3; ++k) c_out[3 * q + k] = L.c[q][k];
+        w_out[q] = (double)L.wnum[q] / (double)L.wden[q];
+        opp_out[q] = L.opp[q];
+    }
+    return 0;
+}
+
+size_t orc_addr(const lbm_desc* d, int x, int y, int z, int k, int K) { return make_geo(d).addr(x, y, z, k, K); }
+
+#define ORC_DISPATCH(CALL_F32, CALL_F16C, CALL_F64)                   \
+    if (check(d)) return -1;                                          \
+    switch (d->storage) {                                             \
+        case LBM_STORE_F32: return CALL_F32;                          \
+        case LBM_STORE_F16C: return CALL_F16C;                        \
+        case LBM_STORE_F64: return CALL_F64;                          \
+        default: return -1;                                           \
+    }
+
+int orc_step_db(const lbm_desc* d, void* f_out, const void* f_in, const void* bc, const uint8_t* flags, double tau) {
+    ORC_DISPATCH((step_impl<float, float>(d, (float*)f_out, (const float*)f_in, (const float*)bc, flags, tau, 0)),
+                 (step_impl<uint16_t, float>(d, (uint16_t*)f_out, (const uint16_t*)f_in, (const float*)bc, flags, tau, 0)),
+                 (step_impl<double, double>(d, (double*)f_out, (const double*)f_in, (const double*)bc, flags, tau, 0)))
+}
+
+int orc_step_ep(const lbm_desc* d, void* f, const void* bc, const uint8_t* flags, double tau, int is_even) {
+    int mode = is_even ? 1 : 2;
+    ORC_DISPATCH((step_impl<float, float>(d, (float*)f, (const float*)f, (const float*)bc, flags, tau, mode)),
+                 (step_impl<uint16_t, float>(d, (uint16_t*)f, (const uint16_t*)f, (const float*)bc, flags, tau, mode)),
+                 (step_impl<double, double>(d, (double*)f, (const double*)f, (const double*)bc, flags, tau, mode)))
+}
+
+int orc_macroscopic(const lbm_desc* d, void* rho, void* u, const void* f, const void* bc, const uint8_t* flags,
+                    int after_odd, int u_layout) {
+    int mode = d->method == LBM_ESOTERIC_PULL ? (after_odd ? 1 : 2) : 0;
+    ORC_DISPATCH((macro_impl<float, float>(d, (float*)rho, (float*)u, (const float*)f, (const float*)bc, flags, mode, u_layout)),
+                 (macro_impl<uint16_t, float>(d, (float*)rho, (float*)u, (const uint16_t*)f, (const float*)bc, flags, mode, u_layout)),
+                 (macro_impl<double, double>(d, (double*)rho, (double*)u, (const double*)f, (const double*)bc, flags, mode, u_layout)))
+}
+
+int orc_reduce_moments(const lbm_desc* d, double* out, const void* f, const void* bc, const uint8_t* flags,
+                       int after_odd) {
+    int mode = d->method == LBM_ESOTERIC_PULL ? (after_odd ? 1 : 2) : 0;
+    ORC_DISPATCH((reduce_impl<float, float>(d, out, (const float*)f, (const float*)bc, flags, mode)),
+                 (reduce_impl<uint16_t, float>(d, out, (const uint16_t*)f, (const float*)bc, flags, mode)),
+                 (reduce_impl<double, double>(d, out, (const double*)f, (const double*)bc, flags, mode)))
+}
+
+int orc_init_equilibrium(const lbm_desc* d, void* f, double rho, const double* u) {
+    ORC_DISPATCH((init_impl<float, float>(d, (float*)f, rho, u)),
+                 (init_impl<uint16_t, float>(d, (uint16_t*)f, rho, u)),
+                 (init_impl<double, double>(d, (double*)f, rho, u)))
+}
+
+/* set_exterior_walls host loop, src/lbm/preprocess/boundary_condition.mojo:78-183 (no validation here;
+ * the validation restatement lives in the tests). bc in the grid float type. */
+int orc_set_exterior_wall(const lbm_desc* d, uint8_t* flags, void* bc, int axis, int plus, int flag,
+                          const double* u, double rho) {
+    if (check(d)) return -1;
+    Geo g = make_geo(d);
+    int D = d->D;
+    int n[3] = {g.nx, g.ny, g.nz};
+    int fixed = plus ? n[axis] - 1 : 0;
+    int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+    for (int i = 0; i < n[a1]; ++i)
+        for (int j = 0; j < n[a2]; ++j) {
+            int idx[3];
+            idx[axis] = fixed; idx[a1] = i; idx[a2] = j;
+            flags[g.addr(idx[0], idx[1], idx[2], 0, 1)] = (uint8_t)flag;
+            for (int k = 0; k <= D; ++k) {
+                double v = (k < D) ? (u ? u[k] : NAN) : rho;
+                size_t a = g.addr(idx[0], idx[1], idx[2], k, D + 1);
+                if (d->storage == LBM_STORE_F64) ((double*)bc)[a] = v;
+                else ((float*)bc)[a] = (float)v;
+            }
+        }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+"""ctypes wrapper of the CPU oracle (oracle/lbm_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker. The product path never uses it.
+Arrays are numpy; f holds float32, float64 or uint16 Float16C codes, bc the grid
+float type (float32 for F32/F16C, float64 for F64).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from mojo_lbm_amd._lib import LbmDesc
+
+LIB_PATH = Path(__file__).resolve().parent / "_build" / "liblbm_oracle.so"
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            from oracle.build import build
+
+            build(verbose=False)
+        L = C.CDLL(str(LIB_PATH))
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        pd = C.POINTER(LbmDesc)
+        L.orc_set_threads.argtypes = [i]
+        L.orc_max_threads.restype = i
+        L.orc_f16c_encode.argtypes = [C.c_float]
+        L.orc_f16c_encode.restype = C.c_uint16
+        L.orc_f16c_decode.argtypes = [C.c_uint16]
+        L.orc_f16c_decode.restype = C.c_float
+        L.orc_f16c_encode_n.argtypes = [vp, vp, C.c_size_t]
+        L.orc_f16c_decode_n.argtypes = [vp, vp, C.c_size_t]
+        L.orc_lattice.argtypes = [i, i, vp, vp, vp]
+        L.orc_lattice.restype = i
+        L.orc_addr.argtypes = [pd, i, i, i, i, i]
+        L.orc_addr.restype = C.c_size_t
+        L.orc_step_db.argtypes = [pd, vp, vp, vp, vp, d]
+        L.orc_step_ep.argtypes = [pd, vp, vp, vp, d, i]
+        L.orc_macroscopic.argtypes = [pd, vp, vp, vp, vp, vp, i, i]
+        L.orc_reduce_moments.argtypes = [pd, vp, vp, vp, vp, i]
+        L.orc_init_equilibrium.argtypes = [pd, vp, d, vp]
+        L.orc_set_exterior_wall.argtypes = [pd, vp, vp, i, i, i, vp, d]
+        for n in ("orc_step_db", "orc_step_ep", "orc_macroscopic", "orc_reduce_moments",
+                  "orc_init_equilibrium", "orc_set_exterior_wall"):
+            getattr(L, n).restype = i
+        L.orc_set_threads(L.orc_max_threads())
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _ok(rc: int, what: str):
+    if rc != 0:
+        raise ValueError(f"oracle {what} rejected the description")
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return lib().orc_max_threads()
+
+
+def f_dtype(desc: LbmDesc):
+    return {0: np.float32, 1: np.uint16, 2: np.float64}[desc.storage]
+
+
+def r_dtype(desc: LbmDesc):
+    return np.float64 if desc.storage == 2 else np.float32
+
+
+def n_cells(desc: LbmDesc) -> int:
+    return desc.nx * desc.ny * desc.nz
+
+
+def f16c_encode(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, np.uint16)
+    lib().orc_f16c_encode_n(_p(x), _p(out), x.size)
+    return out
+
+
+def f16c_decode(h: np.ndarray) -> np.ndarray:
+    h = np.ascontiguousarray(h, dtype=np.uint16)
+    out = np.empty(h.shape, np.float32)
+    lib().orc_f16c_decode_n(_p(h), _p(out), h.size)
+    return out
+
+
+def lattice(D: int, Q: int):
+    c = np.zeros((Q, 3), np.int32)
+    w = np.zeros(Q, np.float64)
+    opp = np.zeros(Q, np.int32)
+    _ok(lib().orc_lattice(D, Q, _p(c), _p(w), _p(opp)), "lattice")
+    return c, w, opp
+
+
+def addr(desc: LbmDesc, x: int, y: int, z: int, k: int, K: int) -> int:
+    return lib().orc_addr(C.byref(desc), x, y, z, k, K)
+
+
+def step_db(desc, f_out, f_in, bc, flags, tau):
+    _ok(lib().orc_step_db(C.byref(desc), _p(f_out), _p(f_in), _p(bc), _p(flags), float(tau)), "step_db")
+
+
+def step_ep(desc, f, bc, flags, tau, is_even):
+    _ok(lib().orc_step_ep(C.byref(desc), _p(f), _p(bc), _p(flags), float(tau), int(bool(is_even))), "step_ep")
+
+
+def macroscopic(desc, f, bc, flags, after_odd=False, u_layout=0):
+    n = n_cells(desc)
+    rho = np.zeros(n, r_dtype(desc))
+    u = np.zeros(n * desc.D, r_dtype(desc))
+    _ok(lib().orc_macroscopic(C.byref(desc), _p(rho), _p(u), _p(f), _p(bc), _p(flags), int(after_odd),
+                              int(u_layout)), "macroscopic")
+    return rho, u
+
+
+def reduce_moments(desc, f, bc, flags, after_odd=False):
+    out = np.zeros(desc.D + 1, np.float64)
+    _ok(lib().orc_reduce_moments(C.byref(desc), _p(out), _p(f), _p(bc), _p(flags), int(after_odd)), "reduce")
+    return out
+
+
+def init_equilibrium(desc, f, rho=1.0, u=None):
+    ua = None if u is None else np.ascontiguousarray(u, np.float64)
+    _ok(lib().orc_init_equilibrium(C.byref(desc), _p(f), float(rho), None if ua is None else _p(ua)), "init")
+
+
+def set_exterior_wall(desc, flags, bc, side: str, flag: int, u=None, rho=float("nan")):
+    axis = "XYZ".index(side[1])
+    ua = None if u is None else np.ascontiguousarray(u, np.float64)
+    _ok(lib().orc_set_exterior_wall(C.byref(desc), _p(flags), _p(bc), axis, int(side[0] == "+"), int(flag),
+                                    None if ua is None else _p(ua), float(rho)), "set_exterior_wall")
+
+
+def alloc(desc):
+    """Zeroed f, bc (NaN-free zeros) and flags arrays for a description."""
+    n = n_cells(desc)
+    f = np.zeros(n * desc.Q, f_dtype(desc))
+    bc = np.zeros(n * (desc.D + 1), r_dtype(desc))
+    flags = np.zeros(n, np.uint8)
+    return f, bc, flags

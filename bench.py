@@ -39,7 +39,7 @@ def algorithmic_bytes(Q, nx, ny, nz, s, walls=True):
     return n_f * 2 * Q * s + n_c
 
 
-def make_run(desc, u_lid):
+def make_run(desc, u_lid, walls=True):
     import torch
 
     from mojo_lbm_amd import _lib
@@ -57,9 +57,10 @@ def make_run(desc, u_lid):
     D = desc.D
     lid = _lib.dvec([u_lid] + [0.0] * (D - 1))
     zero = _lib.dvec([0.0] * D)
-    _lib.check(L.lbm_set_exterior_wall(h.ptr, flags.data_ptr(), bc.data_ptr(), b"+Y", 1, lid, 1.0, st))
-    for side in (b"-Y", b"+X", b"-X", b"-Z", b"+Z")[: (3 if D == 2 else 5)]:
-        _lib.check(L.lbm_set_exterior_wall(h.ptr, flags.data_ptr(), bc.data_ptr(), side, 1, zero, 1.0, st))
+    if walls:
+        _lib.check(L.lbm_set_exterior_wall(h.ptr, flags.data_ptr(), bc.data_ptr(), b"+Y", 1, lid, 1.0, st))
+        for side in (b"-Y", b"+X", b"-X", b"-Z", b"+Z")[: (3 if D == 2 else 5)]:
+            _lib.check(L.lbm_set_exterior_wall(h.ptr, flags.data_ptr(), bc.data_ptr(), side, 1, zero, 1.0, st))
     torch.cuda.synchronize()
     return dict(h=h, f=f, f2=f2, bc=bc, flags=flags, L=L, st=st, desc=desc)
 
@@ -159,7 +160,21 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default=None, help="run one variant: fp32 | f16c | f16c_ep | fp32_ep")
+    ap.add_argument("--xcd-remap", type=int, default=None, help="override LBM_OPT_XCD_REMAP (0 or 1)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run only a plain 1 GiB -> 1 GiB device copy kernel (for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.calibrate:
+        import torch
+
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "torch.mul(x, 1.0, out=y) over 2^28 float32",
+                          "read_bytes": 1 << 30, "write_bytes": 1 << 30}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (every box has one MI355X). "
@@ -180,20 +195,27 @@ def main():
                        smagorinsky_cs=0.1)
 
     variants = {
-        "fp32": (0, 0, 0, 4),      # storage, method, ddf, bytes per population
-        "f16c": (1, 0, 1, 2),
-        "f16c_ep": (1, 1, 1, 2),
-        "fp32_ep": (0, 1, 0, 4),
+        "fp32": (0, 0, 0, 4, True),      # storage, method, ddf, bytes per population, walls
+        "f16c": (1, 0, 1, 2, True),
+        "f16c_ep": (1, 1, 1, 2, True),
+        "fp32_ep": (0, 1, 0, 4, True),
+        # the reference's own benchmark box: no walls, fully periodic (kernels/benchmark/double_buffer.mojo:88-91)
+        "fp32_periodic": (0, 0, 0, 4, False),
+        "f16c_periodic": (1, 0, 1, 2, False),
     }
     names = [args.only] if args.only else list(variants)
     results = {}
     for name in names:
-        storage, method, ddf, s = variants[name]
+        storage, method, ddf, s, walls = variants[name]
         d = desc_for(storage, method, ddf)
-        run = make_run(d, u_lid)
+        run = make_run(d, u_lid, walls)
+        if args.xcd_remap is not None:
+            from mojo_lbm_amd import _lib as _L
+
+            _L.check(run["L"].lbm_set_option(run["h"].ptr, _L.OPT_XCD_REMAP, args.xcd_remap))
         steps = args.steps if (name == "fp32" or args.only) else max(100, args.steps // 2)
         t = time_run(run, tau, steps, args.warmup)
-        B = algorithmic_bytes(19, N, N, N, s)
+        B = algorithmic_bytes(19, N, N, N, s, walls)
         ms = t / steps * 1e3
         results[name] = {
             "ms_per_step": ms,

@@ -169,6 +169,25 @@ int lbm_init_equilibrium(lbm_handle* h, void* f, double rho, const double* u, vo
 int lbm_set_exterior_wall(lbm_handle* h, uint8_t* flags, void* bc, const char* side,
                           int flag, const double* u, double rho, void* stream);
 
+/*
+ * Float16C codec on the device for n values: to_f16c = 1 converts floats to
+ * uint16 codes, 0 converts codes to floats. The same codec the F16C storage
+ * uses (Float16C.to_fp16c / to_fp32, src/utils/custom_fp.mojo:36-100), for
+ * converting fields in and out of the population format.
+ */
+int lbm_f16c_convert(const void* in, void* out, size_t n, int to_f16c, void* stream);
+
+/*
+ * Launch tuning of one handle (no reference counterpart: the reference fixes
+ * the launch shape at compile time, src/lbm/grid.mojo:99-100).
+ *   LBM_OPT_XCD_REMAP: 1 = give each of the 8 XCDs a contiguous run of tiles
+ *                      (default when the tile count is a multiple of 8),
+ *                      0 = hardware round-robin order.
+ * Returns LBM_ERR_INVALID for an unknown option or value.
+ */
+enum { LBM_OPT_XCD_REMAP = 1 };
+int lbm_set_option(lbm_handle* h, int option, int value);
+
 /* Device memory helpers for hosts without their own GPU runtime (the Mojo host
  * of ContextTileTensor, src/utils/contextTileTensor.mojo:113-158, 280-293). */
 int lbm_alloc(size_t bytes, void** ptr);

@@ -17,6 +17,7 @@ COLL_SRT, COLL_TRT, COLL_KBC, COLL_RLBM = 0, 1, 2, 3
 STORE_F32, STORE_F16C, STORE_F64 = 0, 1, 2
 LES_NONE, LES_SMAGORINSKY = 0, 1
 U_XYZD, U_DXYZ = 0, 1
+OPT_XCD_REMAP = 1
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NO_DEVICE = 0, -1, -2, -3, -4
 
 
@@ -57,6 +58,8 @@ SIGNATURES = {
     "lbm_reduce_moments_f64": ([_vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
     "lbm_init_equilibrium": ([_vp, _vp, _d, C.POINTER(_d), _vp], _i),
     "lbm_set_exterior_wall": ([_vp, _vp, _vp, _cp, _i, C.POINTER(_d), _d, _vp], _i),
+    "lbm_f16c_convert": ([_vp, _vp, _sz, _i, _vp], _i),
+    "lbm_set_option": ([_vp, _i, _i], _i),
     "lbm_alloc": ([_sz, C.POINTER(_vp)], _i),
     "lbm_free": ([_vp], _i),
     "lbm_memcpy_h2d": ([_vp, _vp, _sz, _vp], _i),

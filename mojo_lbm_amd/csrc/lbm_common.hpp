@@ -74,29 +74,34 @@ template <class L, class R> __host__ __device__ __forceinline__ constexpr R weig
 //   e' = 0:  |v| = m * 2^-25        e' > 0: |v| = (2048 + m) * 2^(e' - 26)
 // Encoding rounds to nearest with ties away from zero and saturates to 0x7FFF
 // for magnitudes that round to 2 or more (and for inf / NaN).
+// Decode: the 15 magnitude bits placed at fp32 bit 12 read as an fp32 number
+// equal to the value times 2^-112 (e' > 0: exponent e' - 127 instead of
+// e' - 15; e' = 0: the fp32 denormal m * 2^-137 instead of m * 2^-25), so one
+// exact multiply by 2^112 restores it. Needs fp32 denormal inputs, which the
+// library keeps (IEEE mode, no flush).
 __device__ __forceinline__ float f16c_decode(uint32_t h) {
-    const uint32_t e = (h >> 11) & 0xFu;
-    const uint32_t m = h & 0x7FFu;
-    const uint32_t sig = e ? (m | 0x800u) : m;       // integer significand
-    const uint32_t k = e ? e : 1u;                   // grid exponent: 2^(k - 26)
-    const float scale = __uint_as_float((k + 101u) << 23);  // 2^(k-26) as an fp32 normal
-    const float mag = (float)sig * scale;            // exact (<= 12 significant bits)
-    return __uint_as_float(__float_as_uint(mag) | ((h & 0x8000u) << 16));
+    const uint32_t bits = ((h & 0x7FFFu) << 12) | ((h & 0x8000u) << 16);
+    return __uint_as_float(bits) * 0x1p112f;
 }
 
+// Encode: in the normal range adding half a code ulp (2^11 in fp32 bits)
+// and truncating 12 bits rounds half away from zero, and subtracting 112 << 11
+// rebiases the exponent; below it the magnitude times 2^25 (exact) counts
+// subnormal code steps, t = r + frac with r = floor(t) and frac = t - r both
+// exact, and frac >= 1/2 rounds half up. (floor(t + 0.5) would not do: the
+// sum itself can round up to the next integer.) Magnitudes of 2 - 2^-12 and
+// above, infinities and NaN saturate to 0x7FFF.
 __device__ __forceinline__ uint32_t f16c_encode(float x) {
     const uint32_t bits = __float_as_uint(x);
     const uint32_t s = (bits >> 16) & 0x8000u;
-    const float a = __uint_as_float(bits & 0x7FFFFFFFu);
-    // Grid exponent E = max(floor(log2 a), -14); t = a * 2^(11 - E) lies in [2^11, 2^12)
-    // (or [0, 2^11) below the normal range); floor(t + 0.5) is exact round-half-up.
-    int E = (int)((bits >> 23) & 0xFFu) - 127;
-    E = E < -14 ? -14 : E;
-    const float t = fminf(a * __uint_as_float((uint32_t)(127 + 11 - E) << 23), 8192.0f);  // NaN -> 8192
-    const uint32_t r = (uint32_t)floorf(t + 0.5f);
-    uint32_t code = ((uint32_t)(E + 14) << 11) + r;
-    code = (a < 2.0f) ? code : 0x7FFFu;  // saturation; NaN compares false
-    code = code > 0x7FFFu ? 0x7FFFu : code;
+    const uint32_t a = bits & 0x7FFFFFFFu;
+    const uint32_t rn = a + 0x800u;
+    const uint32_t normal = (rn >> 12) - (112u << 11);
+    const float t = fminf(__uint_as_float(a) * 0x1p25f, 4096.0f);
+    const float r = floorf(t);
+    const uint32_t sub = (uint32_t)r + (t - r >= 0.5f ? 1u : 0u);
+    uint32_t code = rn >= (113u << 23) ? normal : sub;
+    code = a >= 0x3FFFF800u ? 0x7FFFu : code;
     return s | code;
 }
 
@@ -136,6 +141,8 @@ struct Geo {
     int tile_cells;       // tx * ty * tz
     int n_tiles;
     int xcd_chunk;        // tiles per XCD when the XCD-contiguous remap is on, else 0
+    uint32_t hx_magic;    // floor(2^32 / (tx + 2)) + 1: n / (tx + 2) == umulhi(n, hx_magic) for n < 2^16
+    uint32_t hxy_magic;   // the same for (tx + 2) * (ty + 2)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X. With the

@@ -2,29 +2,38 @@
 //
 // One workgroup per layout tile, one lane per node; lane id == the node's
 // column-major index inside the tile, so every population plane of a tile is
-// read and written as contiguous 64-lane runs. The tile's flags plus a one-node
-// halo are staged in LDS; a tile whose flags and halo are all FLUID takes a
-// branch-free periodic pull path. Collision arithmetic follows the reference's
-// operation order (moments summed in q order, u = sum/rho, 1/tau, f_eq as
-// written) and the library is built with -ffp-contract=off, so FP32 and FP64
-// results are rounded exactly like the CPU restatement.
+// read and written as contiguous 64-lane runs.
+//
+// A pre-pass (tile_summary_kernel) ORs the flags of every tile. A tile whose
+// own and 26 neighbouring tiles hold only FLUID nodes is an interior tile: it
+// pulls every population from x - c_q and collides, with no flag traffic, no
+// LDS and no barrier. Other tiles stage their flags plus a one-node halo in
+// LDS and apply bounce-back, moving walls and equilibrium nodes.
+//
+// Populations are addressed through buffer resources: each lane carries a
+// 32-bit byte offset and the q-plane offset q * tile_cells rides in the scalar
+// soffset operand, so a load or store costs at most one vector add. Buffers of
+// 4 GB or more use a 64-bit pointer variant (BIG).
 #pragma once
 
-#include "lbm_common.hpp"
+#include "lbm_physics.hpp"
 
 namespace lbm {
 
 enum Mode { MODE_DB = 0, MODE_EP_EVEN = 1, MODE_EP_ODD = 2 };
 
 template <class R> struct StepParams {
-    const void* f_in;   // typed by the storage
-    void* f_out;        // == f_in for esoteric pull
+    const void* f_in;          // typed by the storage
+    void* f_out;               // == f_in for esoteric pull
     const R* bc;
     const uint8_t* flags;
+    const uint32_t* summary;   // per-tile OR of the flags; NULL: every tile takes the boundary path
     R tau;
+    R inv_tau;                 // 1/tau rounded in R
     R cs;
-    int moving;         // include_moving_boundary
-    int eqbc;           // EQUILIBRIUM in the included BCs
+    int moving;                // include_moving_boundary
+    int eqbc;                  // EQUILIBRIUM in the included BCs
+    uint32_t f_bytes;          // bytes of one f buffer (32-bit addressing only)
     Geo g;
 };
 
@@ -34,266 +43,126 @@ template <class R> struct MacroParams {
     const uint8_t* flags;
     R* rho;
     R* u;
-    double* partial;    // reduction mode: per-tile (D+1) sums
+    double* partial;           // reduction mode: per-tile (D+1) sums
     int u_layout;
     int moving;
     int eqbc;
+    uint32_t f_bytes;
     Geo g;
 };
 
-// ------------------------------------------------------------ moment helpers
-// Sums skip the terms whose lattice coefficient is zero: the reference adds
-// f*0 = +-0 there (Vector.dot / Vector.sum, src/utils/vector.mojo:205-224),
-// which leaves every nonzero partial sum unchanged.
-template <class L, class R>
-__host__ __device__ __forceinline__ R dot_c(const R (&u)[3], int q) {
-    R out = 0;
-    bool first = true;
-#pragma unroll
-    for (int d = 0; d < L::D; ++d) {
-        const int c = L::c(q, d);
-        if (c == 0) continue;
-        const R t = c > 0 ? u[d] : -u[d];
-        out = first ? t : out + t;
-        first = false;
+// ------------------------------------------------------------ tile region masks
+// Bit (sx+1) + 3 (sy+1) + 9 (sz+1), s in {-1, 0, 1}, of a tile's mask is set
+// when a non-FLUID node lies in region (sx, sy, sz): s_d = 0 is the whole
+// extent along axis d, s_d = -1 / +1 the first / last layer. The halo a tile
+// reads from its neighbour at offset (dx, dy, dz) is exactly that neighbour's
+// region (-dx, -dy, -dz); the tile's own nodes are region (0, 0, 0).
+__host__ __device__ constexpr uint32_t region_bit(int sx, int sy, int sz) {
+    return 1u << ((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1));
+}
+// All regions with s_d == s along axis d.
+__host__ __device__ constexpr uint32_t axis_regions(int d, int s) {
+    uint32_t m = 0;
+    for (int sz = -1; sz <= 1; ++sz)
+        for (int sy = -1; sy <= 1; ++sy)
+            for (int sx = -1; sx <= 1; ++sx)
+                if ((d == 0 ? sx : d == 1 ? sy : sz) == s) m |= region_bit(sx, sy, sz);
+    return m;
+}
+
+// ------------------------------------------------------------ population access
+// Cache policy of population stores: bit 1 (nt) streams them past the caches;
+// a step never rereads what it writes.
+constexpr int kStoreNT = 2;
+
+// Raw bits <-> compute value for each storage.
+template <class S> struct Bits;
+template <> struct Bits<StoreF32> {
+    __device__ __forceinline__ static float get(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
     }
-    return out;
-}
-
-template <class L, class R>
-__host__ __device__ __forceinline__ R dot_uu(const R (&u)[3]) {
-    R out = u[0] * u[0];
-#pragma unroll
-    for (int d = 1; d < L::D; ++d) out = out + u[d] * u[d];
-    return out;
-}
-
-// f_eq, src/lbm/kernels/utils/equilibrium.mojo:10-38.
-template <class L, class R, bool DDF>
-__host__ __device__ __forceinline__ R feq(int q, R rho, const R (&u)[3], R uu) {
-    const R eu = dot_c<L, R>(u, q);
-    const R w = weight<L, R>(q);
-    if (DDF) return w * rho * ((R)3 * eu + (R)4.5 * eu * eu - (R)1.5 * uu) + w * (rho - (R)1);
-    return w * rho * ((R)1 + (R)3 * eu + (R)4.5 * eu * eu - (R)1.5 * uu);
-}
-
-// get_density (moment.mojo:17-41) and get_velocity (moment.mojo:45-76).
-template <class L, class R, bool DDF>
-__device__ __forceinline__ R density(const R (&f)[L::Q]) {
-    R s = f[0];
-#pragma unroll
-    for (int q = 1; q < L::Q; ++q) s = s + f[q];
-    return DDF ? s + (R)1 : s;
-}
-
-template <class L, class R>
-__device__ __forceinline__ void velocity(const R (&f)[L::Q], R rho, R (&u)[3]) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) u[d] = 0;
-#pragma unroll
-    for (int d = 0; d < L::D; ++d) {
-        bool first = true;
-        R acc = 0;
-#pragma unroll
-        for (int q = 0; q < L::Q; ++q) {
-            const int c = L::c(q, d);
-            if (c == 0) continue;
-            const R t = c > 0 ? f[q] : -f[q];
-            acc = first ? t : acc + t;
-            first = false;
-        }
-        u[d] = acc / rho;
+    __device__ __forceinline__ static void put(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, v, s, kStoreNT);
     }
-}
-
-__host__ __device__ constexpr int stress_a(int D, int n) { return D == 2 ? (n < 2 ? 0 : 1) : (n < 3 ? 0 : (n < 5 ? 1 : 2)); }
-__host__ __device__ constexpr int stress_b(int D, int n) {
-    return D == 2 ? (n == 0 ? 0 : 1) : (n == 0 ? 0 : (n == 1 ? 1 : (n == 2 ? 2 : (n == 3 ? 1 : 2))));
-}
-
-// get_non_eq_second_order_moment / get_Qiab, moment.mojo:80-150.
-template <class L, class R>
-__device__ __forceinline__ void pi_neq(const R (&fn)[L::Q], R (&pi)[6]) {
-    constexpr int NS = L::D * (L::D + 1) / 2;
-#pragma unroll
-    for (int n = 0; n < NS; ++n) {
-        const int a = stress_a(L::D, n), b = stress_b(L::D, n);
-        bool first = true;
-        R acc = 0;
-#pragma unroll
-        for (int q = 0; q < L::Q; ++q) {
-            const int cc = L::c(q, a) * L::c(q, b);
-            if (cc == 0) continue;
-            const R t = cc > 0 ? fn[q] : -fn[q];
-            acc = first ? t : acc + t;
-            first = false;
-        }
-        pi[n] = acc;
+};
+template <> struct Bits<StoreF16C> {
+    __device__ __forceinline__ static float get(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return f16c_decode(__builtin_amdgcn_raw_buffer_load_b16(r, v, s, 0));
     }
-}
-
-// Smagorinsky eddy relaxation time: get_strain_rate_tensor (moment.mojo:154-176),
-// its squared norm (:182-215) and get_Smagorinsky_LES_tau (ops/turbulence.mojo:12-38).
-// The strain rate uses the base tau (steps/collide.mojo:69).
-template <class L, class R>
-__device__ __forceinline__ R smagorinsky_tau(const R (&pi)[6], R rho, R tau, R cs) {
-    constexpr int NS = L::D * (L::D + 1) / 2;
-    const R cs2 = (R)1 / (R)3;
-    const R k = (R)(-1) / ((R)2 * rho * cs2 * tau);
-    R snorm = 0;
-#pragma unroll
-    for (int n = 0; n < NS; ++n) {
-        const R s = pi[n] * k;
-        const R ss = s * s;
-        snorm = snorm + ((stress_a(L::D, n) != stress_b(L::D, n)) ? ss * (R)2 : ss);
+    __device__ __forceinline__ static void put(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x) {
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f16c_encode(x), r, v, s, kStoreNT);
     }
-    const R v_eddy = (cs * cs) * sqrt((R)2 * snorm);
-    return (R)3 * v_eddy;
-}
-
-// KBC shear part. D2Q9: get_shear_D2Q9 (ops/collisions/KBC_.mojo:137-175).
-// D3Q19 / D3Q27 have no reference code; the same construction is used there:
-// s_q = w_q / (2 cs^4) * sum_ab (c_a c_b - cs^2 delta_ab) dev(Pi)_ab,
-// which reproduces the D2Q9 table term by term.
-template <class L, class R>
-__device__ __forceinline__ void kbc_shear(const R (&pi)[6], R (&s)[L::Q]) {
-    if (L::D == 2) {
-        const R N = pi[0] - pi[2];
-        s[0] = 0;
-        s[1] = N * (R)0.25;
-        s[2] = N * (R)0.25;
-        s[3] = -N * (R)0.25;
-        s[4] = -N * (R)0.25;
-        s[5] = pi[1] * (R)0.25;
-        s[6] = pi[1] * (R)0.25;
-        s[7] = -pi[1] * (R)0.25;
-        s[8] = -pi[1] * (R)0.25;
-        return;
+};
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+template <> struct Bits<StoreF64> {
+    __device__ __forceinline__ static double get(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
     }
-    const R tr3 = (pi[0] + pi[3] + pi[5]) / (R)3;
-    const R dev[3] = {pi[0] - tr3, pi[3] - tr3, pi[5] - tr3};
-    const R off[3] = {pi[1], pi[2], pi[4]};  // xy, xz, yz
-#pragma unroll
-    for (int q = 0; q < L::Q; ++q) {
-        R diag = 0, offs = 0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            if (L::c(q, d) != 0) diag = diag + dev[d];
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-            const int a = n == 2 ? 1 : 0, b = n == 0 ? 1 : 2;
-            const int cc = L::c(q, a) * L::c(q, b);
-            if (cc != 0) offs = offs + (cc > 0 ? off[n] : -off[n]);
-        }
-        s[q] = weight<L, R>(q) * (R)4.5 * (diag + (R)2 * offs);
+    __device__ __forceinline__ static void put(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, double x) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, v, s, kStoreNT);
     }
-}
+};
 
-enum Coll { C_SRT = 0, C_KBC = 2 };
-
-// collide, src/lbm/kernels/steps/collide.mojo:23-90 (SRT: ops/collisions/collision.mojo:12-46,
-// KBC: ops/collisions/KBC_.mojo:180-238).
-template <class L, class R, int COLL, bool LES, bool DDF>
-__device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R cs) {
-    constexpr int Q = L::Q;
-    const R rho = density<L, R, DDF>(f);
-    R u[3];
-    velocity<L, R>(f, rho, u);
-    const R uu = dot_uu<L, R>(u);
-    R tau_local = tau;
-    if (LES || COLL == C_KBC) {
-        R fn[Q];
-        R pi[6];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) fn[q] = f[q] - feq<L, R, DDF>(q, rho, u, uu);
-        pi_neq<L, R>(fn, pi);
-        if (LES) tau_local = tau_local + smagorinsky_tau<L, R>(pi, rho, tau, cs);
-        if (COLL == C_KBC) {
-            const R beta = (R)1 / ((R)2 * tau_local);
-            const R inv_beta = (R)2 * tau_local;
-            R ds[Q];
-            kbc_shear<L, R>(pi, ds);
-            R sp1 = 0, sp2 = 0;
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const R fe = feq<L, R, false>(q, rho, u, uu);
-                const R dh = fn[q] - ds[q];
-                sp1 = sp1 + ds[q] * dh / fe;
-                sp2 = sp2 + dh * dh / fe;
-            }
-            const R gamma = inv_beta - ((R)2 - inv_beta) * sp1 / (sp2 + (R)1e-32);
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const R dh = fn[q] - ds[q];
-                f[q] = f[q] - beta * (ds[q] * (R)2 + gamma * dh);
-            }
-            return;
-        }
+// One population field. ld/st take the lane's element index (node part) and a
+// uniform element offset (the q plane).
+template <class S, bool BIG> struct Field {
+    using T = typename S::T;
+    using R = typename S::R;
+    T* ptr;
+    __amdgpu_buffer_rsrc_t rsrc;
+    __device__ __forceinline__ Field(const void* p, uint32_t bytes) : ptr((T*)p) {
+        if (!BIG) rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, (int)bytes, 0x00020000);
     }
-    const R inv_tau = (R)1 / tau_local;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = f[q] - inv_tau * (f[q] - feq<L, R, DDF>(q, rho, u, uu));
-}
+    __device__ __forceinline__ R ld(uint32_t lane, uint32_t uni) const {
+        if (BIG) return S::ld(ptr + ((size_t)lane + uni));
+        return Bits<S>::get(rsrc, lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T));
+    }
+    __device__ __forceinline__ void st(uint32_t lane, uint32_t uni, R v) const {
+        if (BIG) S::st(ptr + ((size_t)lane + uni), v);
+        else Bits<S>::put(rsrc, lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T), v);
+    }
+};
 
 // ------------------------------------------------------------ tile context
-// Per-lane position, LDS flag tile with halo, and the element deltas of a one
-// node move along each axis (periodic, get_adjacent_idx src/lbm/kernels/utils/index.mojo:44-64).
+// Per-lane position and the element deltas of a one-node move along each axis
+// (periodic, get_adjacent_idx src/lbm/kernels/utils/index.mojo:44-64), plus,
+// for boundary tiles, the LDS flag tile with its one-node halo. Halo flags are
+// fetched into registers before the population loads are issued and written
+// to LDS after, so their latency overlaps the population traffic.
 template <class L> struct TileCtx {
+    static constexpr int MAXR = 8;  // halo rounds per lane (hcells <= 8 * tile_cells, checked on the host)
     int x, y, z;          // node
     int tid;
     int hx, hxy;          // LDS halo strides
     int center;           // own position in the LDS tile
     uint32_t tile;        // tile id
-    int dl_m[3], dl_p[3]; // local-index delta for a -1 / +1 move
-    int dt_m[3], dt_p[3]; // tile-index delta for a -1 / +1 move
+    int bx, by, bz;       // tile coordinates (uniform)
+    uint32_t fbase;       // element index of (x, q = 0) in f
+    uint32_t fm[3], fp[3];// element deltas of a -1 / +1 move in f (mod 2^32)
+    uint32_t tc;          // tile_cells: element distance between q planes
     int nonfluid;         // tile or halo holds a non-FLUID flag
     uint8_t own;
+    uint32_t hv[MAXR];    // fetched halo flags
 
-    __device__ __forceinline__ void init(const Geo& g, const uint8_t* __restrict__ flags, uint8_t* sfl) {
-        constexpr int D = L::D;
-        tile = tile_of_block(g);
+    __device__ __forceinline__ void init_pos(const Geo& g) {
+        constexpr int D = L::D, Q = L::Q;
+        tile = (uint32_t)__builtin_amdgcn_readfirstlane(tile_of_block(g));
         tid = threadIdx.x;
         const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
         const int lx = tid & (tx - 1), ly = (tid >> g.txb) & (ty - 1), lz = tid >> (g.txb + g.tyb);
-        const int bx = (int)(tile % (uint32_t)g.Tx);
+        bx = __builtin_amdgcn_readfirstlane((int)(tile % (uint32_t)g.Tx));
         const int byz = (int)(tile / (uint32_t)g.Tx);
-        const int by = byz % g.Ty, bz = byz / g.Ty;
+        by = __builtin_amdgcn_readfirstlane(byz % g.Ty);
+        bz = __builtin_amdgcn_readfirstlane(byz / g.Ty);
         x = (bx << g.txb) + lx;
         y = (by << g.tyb) + ly;
         z = (bz << g.tzb) + lz;
-
         hx = tx + 2;
-        const int hy = ty + 2;
-        const int hz = D == 3 ? tz + 2 : 1;
-        hxy = hx * hy;
-        const int hcells = hxy * hz;
-        int any = 0;
-        for (int h = tid; h < hcells; h += blockDim.x) {
-            const int iz = h / hxy;
-            const int r = h - iz * hxy;
-            const int iy = r / hx;
-            const int ix = r - iy * hx;
-            int gx = (bx << g.txb) + ix - 1;
-            gx += gx < 0 ? g.nx : 0;
-            gx -= gx >= g.nx ? g.nx : 0;
-            int gy = (by << g.tyb) + iy - 1;
-            gy += gy < 0 ? g.ny : 0;
-            gy -= gy >= g.ny ? g.ny : 0;
-            int gz = 0;
-            if (D == 3) {
-                gz = (bz << g.tzb) + iz - 1;
-                gz += gz < 0 ? g.nz : 0;
-                gz -= gz >= g.nz ? g.nz : 0;
-            }
-            const uint32_t loc = (uint32_t)((gx & (tx - 1)) + ((gy & (ty - 1)) << g.txb) + ((gz & (tz - 1)) << (g.txb + g.tyb)));
-            const uint32_t tl = (uint32_t)((gx >> g.txb) + g.Tx * ((gy >> g.tyb) + g.Ty * (gz >> g.tzb)));
-            const uint8_t fl = flags[tl * (uint32_t)g.tile_cells + loc];
-            sfl[h] = fl;
-            any |= fl;
-        }
-        nonfluid = __syncthreads_or(any);
+        hxy = hx * (ty + 2);
         center = (lz + (D == 3 ? 1 : 0)) * hxy + (ly + 1) * hx + (lx + 1);
-        own = sfl[center];
+        tc = (uint32_t)g.tile_cells;
+        const uint32_t TQ = tc * (uint32_t)Q;
+        fbase = tile * TQ + (uint32_t)tid;
 
         const int Ts[3] = {1, g.Tx, g.Tx * g.Ty};
         const int ls[3] = {1, tx, tx * ty};
@@ -303,34 +172,102 @@ template <class L> struct TileCtx {
         const int Tn[3] = {g.Tx, g.Ty, g.Tz};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            // -1 move
-            if (lc[d] > 0) { dl_m[d] = -ls[d]; dt_m[d] = 0; }
-            else { dl_m[d] = (tn[d] - 1) * ls[d]; dt_m[d] = (bc3[d] > 0 ? -1 : Tn[d] - 1) * Ts[d]; }
-            // +1 move
-            if (lc[d] < tn[d] - 1) { dl_p[d] = ls[d]; dt_p[d] = 0; }
-            else { dl_p[d] = -(tn[d] - 1) * ls[d]; dt_p[d] = (bc3[d] < Tn[d] - 1 ? 1 : -(Tn[d] - 1)) * Ts[d]; }
+            int dl, dt;
+            if (lc[d] > 0) { dl = -ls[d]; dt = 0; }
+            else { dl = (tn[d] - 1) * ls[d]; dt = (bc3[d] > 0 ? -1 : Tn[d] - 1) * Ts[d]; }
+            fm[d] = (uint32_t)dl + (uint32_t)dt * TQ;
+            if (lc[d] < tn[d] - 1) { dl = ls[d]; dt = 0; }
+            else { dl = -(tn[d] - 1) * ls[d]; dt = (bc3[d] < Tn[d] - 1 ? 1 : -(Tn[d] - 1)) * Ts[d]; }
+            fp[d] = (uint32_t)dl + (uint32_t)dt * TQ;
         }
+    }
+
+    // Element index in f (q plane excluded) of node x + s * c_q.
+    __device__ __forceinline__ uint32_t nb(int q, int s) const {
+        uint32_t a = fbase;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int c = s * L::c(q, d);
+            if (c > 0) a += fp[d];
+            if (c < 0) a += fm[d];
+        }
+        return a;
+    }
+
+    // Element index in a field with K values per node of node (xn, yn, zn), value k.
+    __device__ __forceinline__ uint32_t field_index(const Geo& g, int xn, int yn, int zn, int K, int k) const {
+        const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
+        const uint32_t loc = (uint32_t)((xn & (tx - 1)) + ((yn & (ty - 1)) << g.txb) + ((zn & (tz - 1)) << (g.txb + g.tyb)));
+        const uint32_t tl = (uint32_t)((xn >> g.txb) + g.Tx * ((yn >> g.tyb) + g.Ty * (zn >> g.tzb)));
+        return (tl * (uint32_t)K + (uint32_t)k) * tc + loc;
+    }
+
+    // Element index of the pull neighbour x - c_q in a field with K values per node.
+    __device__ __forceinline__ uint32_t pull_index(const Geo& g, int q, int K, int k) const {
+        int xn = x - L::c(q, 0), yn = y - L::c(q, 1), zn = z - L::c(q, 2);
+        xn += xn < 0 ? g.nx : 0;
+        xn -= xn >= g.nx ? g.nx : 0;
+        yn += yn < 0 ? g.ny : 0;
+        yn -= yn >= g.ny ? g.ny : 0;
+        zn += zn < 0 ? g.nz : 0;
+        zn -= zn >= g.nz ? g.nz : 0;
+        return field_index(g, xn, yn, zn, K, k);
+    }
+
+    // Loads the flags of the tile and its halo into registers (no LDS traffic yet).
+    __device__ __forceinline__ void fetch_flags(const Geo& g, const uint8_t* __restrict__ flags) {
+        constexpr int D = L::D;
+        const int tz = 1 << g.tzb;
+        const int hcells = hxy * (D == 3 ? tz + 2 : 1);
+        const int nthr = blockDim.x;
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int h = tid + r * nthr;
+            hv[r] = 0;
+            if (r * nthr >= hcells) break;
+            if (h < hcells) {
+                const int iz = (int)__umulhi((uint32_t)h, g.hxy_magic);
+                const int rr = h - iz * hxy;
+                const int iy = (int)__umulhi((uint32_t)rr, g.hx_magic);
+                const int ix = rr - iy * hx;
+                int gx = (bx << g.txb) + ix - 1;
+                gx += gx < 0 ? g.nx : 0;
+                gx -= gx >= g.nx ? g.nx : 0;
+                int gy = (by << g.tyb) + iy - 1;
+                gy += gy < 0 ? g.ny : 0;
+                gy -= gy >= g.ny ? g.ny : 0;
+                int gz = 0;
+                if (D == 3) {
+                    gz = (bz << g.tzb) + iz - 1;
+                    gz += gz < 0 ? g.nz : 0;
+                    gz -= gz >= g.nz ? g.nz : 0;
+                }
+                hv[r] = flags[field_index(g, gx, gy, gz, 1, 0)];
+            }
+        }
+    }
+
+    // Writes the fetched flags to LDS and synchronises the workgroup.
+    __device__ __forceinline__ void commit_flags(const Geo& g, uint8_t* sfl) {
+        constexpr int D = L::D;
+        const int tz = 1 << g.tzb;
+        const int hcells = hxy * (D == 3 ? tz + 2 : 1);
+        const int nthr = blockDim.x;
+        uint32_t any = 0;
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int h = tid + r * nthr;
+            if (r * nthr >= hcells) break;
+            if (h < hcells) sfl[h] = (uint8_t)hv[r];
+            any |= hv[r];
+        }
+        nonfluid = __syncthreads_or((int)any);
+        own = sfl[center];
     }
 
     // Flag of the pull neighbour x - c_q (LDS).
     __device__ __forceinline__ uint8_t pull_flag(const uint8_t* sfl, int q) const {
         return sfl[center - L::c(q, 0) - L::c(q, 1) * hx - L::c(q, 2) * hxy];
-    }
-
-    // Element index (field with K values per node) of node x + s*c_q, value k.
-    __device__ __forceinline__ uint32_t idx(const Geo& g, int K, int q, int s, int k) const {
-        int dl = 0, dt = 0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int c = s * L::c(q, d);
-            if (c > 0) { dl += dl_p[d]; dt += dt_p[d]; }
-            if (c < 0) { dl += dl_m[d]; dt += dt_m[d]; }
-        }
-        const uint32_t TK = (uint32_t)g.tile_cells * (uint32_t)K;
-        return (tile + (uint32_t)dt) * TK + (uint32_t)(tid + dl) + (uint32_t)k * (uint32_t)g.tile_cells;
-    }
-    __device__ __forceinline__ uint32_t own_idx(const Geo& g, int K, int k) const {
-        return tile * ((uint32_t)g.tile_cells * (uint32_t)K) + (uint32_t)tid + (uint32_t)k * (uint32_t)g.tile_cells;
     }
 };
 
@@ -338,43 +275,107 @@ __device__ __forceinline__ bool node_valid(uint8_t fl) {
     return !(((fl & 0x0F) == LBM_FLAG_SOLID) || (fl & LBM_FLAG_EXCLUDE));  // is_valid_thread
 }
 
-// Gather: stream (pull / esoteric) + moving wall + equilibrium BC, for one valid node.
-// double_buffer_pull_load_f_vec (ops/load_and_store.mojo:164-221), esoteric_pull_load_f_vec (:11-86),
-// moving_wall_bc (ops/boundary_condition.mojo:17-83), equilibrium_bc (:87-204).
-template <class L, class S, int MODE, bool DDF>
-__device__ __forceinline__ void gather(const TileCtx<L>& t, const Geo& g, const uint8_t* sfl,
-                                       const typename S::T* __restrict__ f, const typename S::R* __restrict__ bc,
-                                       int moving, int eqbc, typename S::R (&fv)[L::Q]) {
-    using R = typename S::R;
-    constexpr int Q = L::Q, D = L::D;
-    const bool plain = !t.nonfluid;
-    if (MODE == MODE_DB) {
-        if (plain) {
+// Whether a non-FLUID node (bit 0) or a moving SOLID node (bit 1) lies in the
+// tile or in the one-node halo it reads from its neighbour tiles (periodic):
+// neighbour (dx, dy, dz) contributes its region (-dx, -dy, -dz). Uniform across
+// the workgroup.
+template <int D>
+__device__ __forceinline__ uint32_t boundary_tile(const Geo& g, const uint32_t* __restrict__ summary, int bx, int by,
+                                                  int bz) {
+    uint32_t any = 0, mv = 0;
+    const int zlo = D == 3 ? -1 : 0, zhi = D == 3 ? 1 : 0;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) fv[q] = S::ld(f + t.idx(g, Q, q, -1, q));
-        } else {
+    for (int dz = zlo; dz <= zhi; ++dz) {
+        int tz = bz + dz;
+        tz += tz < 0 ? g.Tz : 0;
+        tz -= tz >= g.Tz ? g.Tz : 0;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const uint8_t nf = q ? t.pull_flag(sfl, q) : t.own;
-                const uint32_t a = (nf == LBM_FLAG_SOLID) ? t.own_idx(g, Q, opp(q)) : t.idx(g, Q, q, -1, q);
-                fv[q] = S::ld(f + a);
+        for (int dy = -1; dy <= 1; ++dy) {
+            int ty = by + dy;
+            ty += ty < 0 ? g.Ty : 0;
+            ty -= ty >= g.Ty ? g.Ty : 0;
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                int tx = bx + dx;
+                tx += tx < 0 ? g.Tx : 0;
+                tx -= tx >= g.Tx ? g.Tx : 0;
+                const uint32_t t = tx + g.Tx * (ty + g.Ty * tz);
+                any |= summary[2 * t] & region_bit(-dx, -dy, -dz);
+                mv |= summary[2 * t + 1] & region_bit(-dx, -dy, -dz);
             }
         }
+    }
+    return (any != 0 ? 1u : 0u) | (mv != 0 ? 2u : 0u);
+}
+
+// Stream, part 1 (issued before the flags are known): the pull of every
+// population from x - c_q (double buffer; double_buffer_pull_load_f_vec,
+// ops/load_and_store.mojo:164-221) or the esoteric slot loads
+// (esoteric_pull_load_f_vec, ops/load_and_store.mojo:11-86). Every address is
+// in bounds (periodic wrap), whatever the flags say.
+template <class L, class S, int MODE, bool BIG>
+__device__ __forceinline__ void load_populations(const TileCtx<L>& t, const Field<S, BIG>& f,
+                                                 typename S::R (&fv)[L::Q]) {
+    constexpr int Q = L::Q;
+    if (MODE == MODE_DB) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fv[q] = f.ld(t.nb(q, -1), (uint32_t)q * t.tc);
     } else {
-        fv[0] = S::ld(f + t.own_idx(g, Q, 0));
+        fv[0] = f.ld(t.fbase, 0);
 #pragma unroll
         for (int p = 1; p < Q; p += 2) {
             const int n = p + 1;
             if (MODE == MODE_EP_EVEN) {
-                fv[p] = S::ld(f + t.own_idx(g, Q, p));
-                fv[n] = S::ld(f + t.idx(g, Q, p, +1, n));
+                fv[p] = f.ld(t.fbase, (uint32_t)p * t.tc);
+                fv[n] = f.ld(t.nb(p, +1), (uint32_t)n * t.tc);
             } else {
-                fv[p] = S::ld(f + t.own_idx(g, Q, n));
-                fv[n] = S::ld(f + t.idx(g, Q, p, +1, p));
+                fv[p] = f.ld(t.fbase, (uint32_t)n * t.tc);
+                fv[n] = f.ld(t.nb(p, +1), (uint32_t)p * t.tc);
             }
         }
     }
-    if (plain) return;
+}
+
+// Store: double buffer writes every population at x; esoteric_pull_store_f_vec
+// (ops/load_and_store.mojo:89-159) writes the parity's slot pattern.
+template <class L, class S, int MODE, bool BIG>
+__device__ __forceinline__ void store_populations(const TileCtx<L>& t, const Field<S, BIG>& f,
+                                                  const typename S::R (&fv)[L::Q]) {
+    constexpr int Q = L::Q;
+    if (MODE == MODE_DB) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f.st(t.fbase, (uint32_t)q * t.tc, fv[q]);
+    } else {
+        f.st(t.fbase, 0, fv[0]);
+#pragma unroll
+        for (int p = 1; p < Q; p += 2) {
+            const int n = p + 1;
+            if (MODE == MODE_EP_EVEN) {
+                f.st(t.nb(p, +1), (uint32_t)n * t.tc, fv[p]);
+                f.st(t.fbase, (uint32_t)p * t.tc, fv[n]);
+            } else {
+                f.st(t.nb(p, +1), (uint32_t)p * t.tc, fv[p]);
+                f.st(t.fbase, (uint32_t)n * t.tc, fv[n]);
+            }
+        }
+    }
+}
+
+// Stream, part 2 (boundary tiles): halfway bounce-back replaces the pull from a
+// SOLID neighbour (raw flag compare, ops/load_and_store.mojo:216) by
+// f(x, opp q), loaded only for those directions; then moving_wall_bc
+// (ops/boundary_condition.mojo:17-83) and equilibrium_bc (:87-204).
+template <class L, class S, int MODE, bool DDF, bool BIG>
+__device__ __forceinline__ void apply_boundaries(const TileCtx<L>& t, const Geo& g, const uint8_t* sfl,
+                                                 const Field<S, BIG>& f, const typename S::R* __restrict__ bc,
+                                                 int moving, int eqbc, typename S::R (&fv)[L::Q]) {
+    using R = typename S::R;
+    constexpr int Q = L::Q, D = L::D;
+    if (MODE == MODE_DB) {
+#pragma unroll
+        for (int q = 1; q < Q; ++q)
+            if (t.pull_flag(sfl, q) == LBM_FLAG_SOLID) fv[q] = f.ld(t.fbase, (uint32_t)opp(q) * t.tc);
+    }
     if (moving) {
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
@@ -382,8 +383,8 @@ __device__ __forceinline__ void gather(const TileCtx<L>& t, const Geo& g, const 
             if ((nf & 0x0F) == LBM_FLAG_SOLID) {
                 R uw[3];
 #pragma unroll
-                for (int d = 0; d < D; ++d) uw[d] = bc[t.idx(g, D + 1, q, -1, d)];
-                const R rw = bc[t.idx(g, D + 1, q, -1, D)];
+                for (int d = 0; d < D; ++d) uw[d] = bc[t.pull_index(g, q, D + 1, d)];
+                const R rw = bc[t.pull_index(g, q, D + 1, D)];
                 fv[q] = fv[q] + (R)6 * weight<L, R>(q) * rw * dot_c<L, R>(uw, q);
             }
         }
@@ -391,8 +392,8 @@ __device__ __forceinline__ void gather(const TileCtx<L>& t, const Geo& g, const 
     if (eqbc && (t.own & 0x0F) == LBM_FLAG_EQUILIBRIUM) {
         R ub[3];
 #pragma unroll
-        for (int d = 0; d < D; ++d) ub[d] = bc[t.own_idx(g, D + 1, d)];
-        const R rb = bc[t.own_idx(g, D + 1, D)];
+        for (int d = 0; d < D; ++d) ub[d] = bc[t.field_index(g, t.x, t.y, t.z, D + 1, d)];
+        const R rb = bc[t.field_index(g, t.x, t.y, t.z, D + 1, D)];
         const int pos[3] = {t.x, t.y, t.z};
         const int nn[3] = {g.nx, g.ny, g.nz};
         R rho = 0, vel[3] = {0, 0, 0};
@@ -429,72 +430,124 @@ __device__ __forceinline__ void gather(const TileCtx<L>& t, const Geo& g, const 
 // ------------------------------------------------------------ kernels
 template <int Q> struct MaxThreads { static constexpr int value = (Q == 9) ? 1024 : 512; };
 
-// double_buffer_kernel (src/lbm/kernels/double_buffer.mojo:25-96) and
-// esoteric_pull_kernel[is_even] (src/lbm/kernels/esoteric_pull.mojo:24-111).
-template <class L, class S, int MODE, int COLL, bool LES, bool DDF>
-__global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParams<typename S::R> p) {
-    using R = typename S::R;
-    using T = typename S::T;
-    constexpr int Q = L::Q;
-    extern __shared__ uint8_t sfl[];
-    TileCtx<L> t;
-    t.init(p.g, p.flags, sfl);
-    if (t.nonfluid && !node_valid(t.own)) return;
-    R fv[Q];
-    gather<L, S, MODE, DDF>(t, p.g, sfl, (const T*)p.f_in, p.bc, p.moving, p.eqbc, fv);
-    collide<L, R, COLL, LES, DDF>(fv, p.tau, p.cs);
-    T* fo = (T*)p.f_out;
-    if (MODE == MODE_DB) {
+// Per-tile region masks of the flags: summary[2 t] marks regions holding a
+// non-FLUID node, summary[2 t + 1] regions holding a SOLID node whose bc
+// velocity is non-zero or whose bc density is not finite (a moving wall: for
+// any other solid the moving-wall term adds exactly +0). Each lane covers 16
+// consecutive nodes of one tile (tile_cells is a power of two in [32, 1024]);
+// a tile is tile_cells / 16 consecutive lanes of one wave, OR-reduced by
+// shuffles. n_chunks = cells / 16 lanes in total.
+template <int D, class R>
+__global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __restrict__ flags,
+                                                           const R* __restrict__ bc, int moving,
+                                                           uint32_t* __restrict__ summary, Geo g,
+                                                           uint32_t n_chunks) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // 16-byte chunk
+    uint32_t mask = 0, mmask = 0;
+    if (i < n_chunks) {
+        const uint4 v = reinterpret_cast<const uint4*>(flags)[i];
+        if ((v.x | v.y | v.z | v.w) != 0u) {
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
+            const uint32_t l0 = (i << 4) & (uint32_t)(g.tile_cells - 1);
+            const uint32_t tile = (i << 4) / (uint32_t)g.tile_cells;
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t fb = (w[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+                if (fb == 0u) continue;
+                const int l = (int)l0 + j;
+                const int lx = l & (tx - 1), ly = (l >> g.txb) & (ty - 1), lz = l >> (g.txb + g.tyb);
+                const uint32_t mx = axis_regions(0, 0) | (lx == 0 ? axis_regions(0, -1) : 0u) |
+                                    (lx == tx - 1 ? axis_regions(0, 1) : 0u);
+                const uint32_t my = axis_regions(1, 0) | (ly == 0 ? axis_regions(1, -1) : 0u) |
+                                    (ly == ty - 1 ? axis_regions(1, 1) : 0u);
+                const uint32_t mz = D == 2 ? axis_regions(2, 0)
+                                           : (axis_regions(2, 0) | (lz == 0 ? axis_regions(2, -1) : 0u) |
+                                              (lz == tz - 1 ? axis_regions(2, 1) : 0u));
+                const uint32_t regs = mx & my & mz;
+                mask |= regs;
+                if (moving && (fb & 0x0Fu) == LBM_FLAG_SOLID) {
+                    const uint32_t tc = (uint32_t)g.tile_cells;
+                    bool mv = false;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) S::st(fo + t.own_idx(p.g, Q, q), fv[q]);
-    } else {
-        // esoteric_pull_store_f_vec, ops/load_and_store.mojo:89-159
-        S::st(fo + t.own_idx(p.g, Q, 0), fv[0]);
-#pragma unroll
-        for (int pq = 1; pq < Q; pq += 2) {
-            const int n = pq + 1;
-            if (MODE == MODE_EP_EVEN) {
-                S::st(fo + t.idx(p.g, Q, pq, +1, n), fv[pq]);
-                S::st(fo + t.own_idx(p.g, Q, pq), fv[n]);
-            } else {
-                S::st(fo + t.idx(p.g, Q, pq, +1, pq), fv[pq]);
-                S::st(fo + t.own_idx(p.g, Q, n), fv[n]);
+                    for (int d = 0; d < D; ++d) mv = mv || bc[(tile * (D + 1) + d) * tc + (uint32_t)l] != (R)0;
+                    const R rw = bc[(tile * (D + 1) + D) * tc + (uint32_t)l];
+                    mv = mv || !isfinite(rw);
+                    if (mv) mmask |= regs;
+                }
             }
         }
     }
+    const uint32_t lanes_per_tile = (uint32_t)g.tile_cells >> 4;  // 2 .. 64
+    for (uint32_t off = 1; off < lanes_per_tile; off <<= 1) {
+        mask |= (uint32_t)__shfl_xor((int)mask, (int)off, 64);
+        mmask |= (uint32_t)__shfl_xor((int)mmask, (int)off, 64);
+    }
+    if ((threadIdx.x & (lanes_per_tile - 1u)) == 0u && i < n_chunks) {
+        summary[2 * (i / lanes_per_tile)] = mask;
+        summary[2 * (i / lanes_per_tile) + 1] = mmask;
+    }
+}
+
+// double_buffer_kernel (src/lbm/kernels/double_buffer.mojo:25-96) and
+// esoteric_pull_kernel[is_even] (src/lbm/kernels/esoteric_pull.mojo:24-111).
+template <class L, class S, int MODE, int COLL, bool LES, bool DDF, bool BIG>
+__global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParams<typename S::R> p) {
+    using R = typename S::R;
+    constexpr int Q = L::Q;
+    extern __shared__ uint8_t sfl[];
+    TileCtx<L> t;
+    t.init_pos(p.g);
+    const uint32_t kind = p.summary ? boundary_tile<L::D>(p.g, p.summary, t.bx, t.by, t.bz) : 3u;
+    const bool boundary = (kind & 1u) != 0;
+    const int moving = p.moving && (kind & 2u);
+    const Field<S, BIG> fin(p.f_in, p.f_bytes);
+    const Field<S, BIG> fout(p.f_out, p.f_bytes);
+    if (boundary) t.fetch_flags(p.g, p.flags);
+    R fv[Q];
+    load_populations<L, S, MODE, BIG>(t, fin, fv);
+    if (boundary) {
+        t.commit_flags(p.g, sfl);
+        if (t.nonfluid) {
+            if (!node_valid(t.own)) return;
+            apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, fin, p.bc, moving, p.eqbc, fv);
+        }
+    }
+    collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs);
+    store_populations<L, S, MODE, BIG>(t, fout, fv);
 }
 
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125) and the FP64 moment
 // reduction: re-stream + BC (tau = 0), moments; solid nodes (raw flag == SOLID) report bc.
-template <class L, class S, int MODE, bool DDF, bool REDUCE>
+template <class L, class S, int MODE, bool DDF, bool REDUCE, bool BIG>
 __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroParams<typename S::R> p) {
     using R = typename S::R;
-    using T = typename S::T;
     constexpr int Q = L::Q, D = L::D;
     extern __shared__ uint8_t sfl[];
     __shared__ double red[D + 1][16];
     TileCtx<L> t;
-    t.init(p.g, p.flags, sfl);
+    t.init_pos(p.g);
+    const Field<S, BIG> f(p.f, p.f_bytes);
+    t.fetch_flags(p.g, p.flags);
+    R fv[Q];
+    load_populations<L, S, MODE, BIG>(t, f, fv);
+    t.commit_flags(p.g, sfl);
+    if (t.nonfluid) apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, f, p.bc, p.moving, p.eqbc, fv);
     R rho = 0, u[3] = {0, 0, 0};
     bool contributes = false;
     if (REDUCE) {
         if (node_valid(t.own)) {
-            R fv[Q];
-            gather<L, S, MODE, DDF>(t, p.g, sfl, (const T*)p.f, p.bc, p.moving, p.eqbc, fv);
             rho = density<L, R, DDF>(fv);
             velocity<L, R>(fv, rho, u);
             contributes = true;
         }
     } else {
         if (t.own != LBM_FLAG_SOLID) {
-            R fv[Q];
-            gather<L, S, MODE, DDF>(t, p.g, sfl, (const T*)p.f, p.bc, p.moving, p.eqbc, fv);
             rho = density<L, R, DDF>(fv);
             velocity<L, R>(fv, rho, u);
         } else {
 #pragma unroll
-            for (int d = 0; d < D; ++d) u[d] = p.bc[t.own_idx(p.g, D + 1, d)];
-            rho = p.bc[t.own_idx(p.g, D + 1, D)];
+            for (int d = 0; d < D; ++d) u[d] = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, d)];
+            rho = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, D)];
         }
         const size_t r = ((size_t)t.x * p.g.ny + t.y) * p.g.nz + t.z;
         const size_t ncell = (size_t)p.g.nx * p.g.ny * p.g.nz;
@@ -563,6 +616,15 @@ __global__ void __launch_bounds__(256) fill_kernel(typename S::T* __restrict__ f
     if (i >= n_elems) return;
     const uint32_t q = (uint32_t)((i / tile_cells) % (uint32_t)Q);
     S::st(f + i, vals.v[q]);
+}
+
+// Float16C codec over a flat array (lbm_f16c_convert).
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(256) f16c_convert_kernel(const void* in, void* out, size_t n, int to_f16c) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (to_f16c) ((uint16_t*)out)[i] = (uint16_t)f16c_encode(((const float*)in)[i]);
+    else ((float*)out)[i] = f16c_decode(((const uint16_t*)in)[i]);
 }
 
 // set_exterior_walls host loop, src/lbm/preprocess/boundary_condition.mojo:160-183.

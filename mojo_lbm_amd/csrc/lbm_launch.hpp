@@ -1,6 +1,7 @@
 // lbm_launch.hpp -- host-side launchers: one Launch<Lattice, Storage> per
 // (velocity set, population storage); each member picks the kernel
-// instantiation for the run's method, collision, LES and DDF toggles.
+// instantiation for the run's method, collision, LES and DDF toggles and the
+// addressing width.
 #pragma once
 
 #include <string>
@@ -13,8 +14,11 @@ struct Handle {
     lbm_desc d;
     Geo g;
     int device = 0;
-    size_t lds_bytes = 0;      // flag tile + halo
-    double* partial = nullptr; // per-tile reduction partials (n_tiles * (D+1))
+    size_t lds_bytes = 0;        // flag tile + halo
+    size_t f_bytes = 0;          // one population buffer
+    bool big = false;            // f_bytes >= 2^31: 64-bit addressing
+    double* partial = nullptr;   // per-tile reduction partials (n_tiles * (D+1))
+    uint32_t* summary = nullptr; // per-tile flag OR, rebuilt before every step
 };
 
 int set_error(int code, const std::string& msg);
@@ -26,8 +30,12 @@ template <class L, class S> struct Launch {
 
     template <int MODE, int COLL, bool LES, bool DDF>
     static int step_k(const Handle& h, const StepParams<R>& p, hipStream_t st) {
-        hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
-                           h.lds_bytes, st, p);
+        if (h.big)
+            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, true>), dim3(h.g.n_tiles),
+                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
+        else
+            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, false>), dim3(h.g.n_tiles),
+                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
         return check_launch("step_kernel");
     }
 
@@ -48,15 +56,23 @@ template <class L, class S> struct Launch {
 
     static int step(const Handle& h, void* f_out, const void* f_in, const void* bc, const uint8_t* flags, double tau,
                     int mode, hipStream_t st) {
+        if ((uintptr_t)flags & 15u) return set_error(LBM_ERR_INVALID, "flags must be 16-byte aligned");
+        const uint32_t n_chunks = (uint32_t)(((size_t)h.g.n_tiles * h.g.tile_cells) >> 4);
+        hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, flags,
+                           (const R*)bc, (int)(h.d.include_moving_boundary != 0), h.summary, h.g, n_chunks);
+        if (int rc = check_launch("tile_summary_kernel")) return rc;
         StepParams<R> p;
         p.f_in = f_in;
         p.f_out = f_out;
         p.bc = (const R*)bc;
         p.flags = flags;
+        p.summary = h.summary;
         p.tau = (R)tau;
+        p.inv_tau = (R)1 / p.tau;
         p.cs = (R)h.d.smagorinsky_cs;
         p.moving = h.d.include_moving_boundary != 0;
         p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
+        p.f_bytes = (uint32_t)(h.big ? 0 : h.f_bytes);
         p.g = h.g;
         if (mode == MODE_DB) return step_mode<MODE_DB>(h, p, st);
         if (mode == MODE_EP_EVEN) return step_mode<MODE_EP_EVEN>(h, p, st);
@@ -65,8 +81,12 @@ template <class L, class S> struct Launch {
 
     template <int MODE, bool DDF, bool REDUCE>
     static int macro_k(const Handle& h, const MacroParams<R>& p, hipStream_t st) {
-        hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, REDUCE>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
-                           h.lds_bytes, st, p);
+        if (h.big)
+            hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, REDUCE, true>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
+                               h.lds_bytes, st, p);
+        else
+            hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, REDUCE, false>), dim3(h.g.n_tiles),
+                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
         return check_launch("macro_kernel");
     }
 
@@ -86,6 +106,7 @@ template <class L, class S> struct Launch {
         p.flags = flags;
         p.moving = h.d.include_moving_boundary != 0;
         p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
+        p.f_bytes = (uint32_t)(h.big ? 0 : h.f_bytes);
         p.g = h.g;
         return p;
     }

@@ -1,0 +1,207 @@
+// lbm_physics.hpp -- per-node LBM arithmetic (moments, f_eq, Smagorinsky, KBC,
+// SRT) in the reference's operation order. Shared by the kernels and by the
+// host-side initialisation (f_eq), and built with -ffp-contract=off so every
+// operation rounds as written.
+#pragma once
+
+#include "lbm_common.hpp"
+
+namespace lbm {
+
+// ------------------------------------------------------------ moment helpers
+// Sums skip the terms whose lattice coefficient is zero: the reference adds
+// f*0 = +-0 there (Vector.dot / Vector.sum, src/utils/vector.mojo:205-224),
+// which leaves every nonzero partial sum unchanged.
+template <class L, class R>
+__host__ __device__ __forceinline__ R dot_c(const R (&u)[3], int q) {
+    R out = 0;
+    bool first = true;
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+        const int c = L::c(q, d);
+        if (c == 0) continue;
+        const R t = c > 0 ? u[d] : -u[d];
+        out = first ? t : out + t;
+        first = false;
+    }
+    return out;
+}
+
+template <class L, class R>
+__host__ __device__ __forceinline__ R dot_uu(const R (&u)[3]) {
+    R out = u[0] * u[0];
+#pragma unroll
+    for (int d = 1; d < L::D; ++d) out = out + u[d] * u[d];
+    return out;
+}
+
+// f_eq, src/lbm/kernels/utils/equilibrium.mojo:10-38.
+template <class L, class R, bool DDF>
+__host__ __device__ __forceinline__ R feq(int q, R rho, const R (&u)[3], R uu) {
+    const R eu = dot_c<L, R>(u, q);
+    const R w = weight<L, R>(q);
+    if (DDF) return w * rho * ((R)3 * eu + (R)4.5 * eu * eu - (R)1.5 * uu) + w * (rho - (R)1);
+    return w * rho * ((R)1 + (R)3 * eu + (R)4.5 * eu * eu - (R)1.5 * uu);
+}
+
+// get_density (moment.mojo:17-41) and get_velocity (moment.mojo:45-76).
+template <class L, class R, bool DDF>
+__device__ __forceinline__ R density(const R (&f)[L::Q]) {
+    R s = f[0];
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) s = s + f[q];
+    return DDF ? s + (R)1 : s;
+}
+
+template <class L, class R>
+__device__ __forceinline__ void velocity(const R (&f)[L::Q], R rho, R (&u)[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = 0;
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+        bool first = true;
+        R acc = 0;
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) {
+            const int c = L::c(q, d);
+            if (c == 0) continue;
+            const R t = c > 0 ? f[q] : -f[q];
+            acc = first ? t : acc + t;
+            first = false;
+        }
+        u[d] = acc / rho;
+    }
+}
+
+__host__ __device__ constexpr int stress_a(int D, int n) { return D == 2 ? (n < 2 ? 0 : 1) : (n < 3 ? 0 : (n < 5 ? 1 : 2)); }
+__host__ __device__ constexpr int stress_b(int D, int n) {
+    return D == 2 ? (n == 0 ? 0 : 1) : (n == 0 ? 0 : (n == 1 ? 1 : (n == 2 ? 2 : (n == 3 ? 1 : 2))));
+}
+
+// get_non_eq_second_order_moment / get_Qiab, moment.mojo:80-150.
+template <class L, class R>
+__device__ __forceinline__ void pi_neq(const R (&fn)[L::Q], R (&pi)[6]) {
+    constexpr int NS = L::D * (L::D + 1) / 2;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const int a = stress_a(L::D, n), b = stress_b(L::D, n);
+        bool first = true;
+        R acc = 0;
+#pragma unroll
+        for (int q = 0; q < L::Q; ++q) {
+            const int cc = L::c(q, a) * L::c(q, b);
+            if (cc == 0) continue;
+            const R t = cc > 0 ? fn[q] : -fn[q];
+            acc = first ? t : acc + t;
+            first = false;
+        }
+        pi[n] = acc;
+    }
+}
+
+// Smagorinsky eddy relaxation time: get_strain_rate_tensor (moment.mojo:154-176),
+// its squared norm (:182-215) and get_Smagorinsky_LES_tau (ops/turbulence.mojo:12-38).
+// The strain rate uses the base tau (steps/collide.mojo:69).
+template <class L, class R>
+__device__ __forceinline__ R smagorinsky_tau(const R (&pi)[6], R rho, R tau, R cs) {
+    constexpr int NS = L::D * (L::D + 1) / 2;
+    const R cs2 = (R)1 / (R)3;
+    const R k = (R)(-1) / ((R)2 * rho * cs2 * tau);
+    R snorm = 0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+        const R s = pi[n] * k;
+        const R ss = s * s;
+        snorm = snorm + ((stress_a(L::D, n) != stress_b(L::D, n)) ? ss * (R)2 : ss);
+    }
+    const R v_eddy = (cs * cs) * sqrt((R)2 * snorm);
+    return (R)3 * v_eddy;
+}
+
+// KBC shear part. D2Q9: get_shear_D2Q9 (ops/collisions/KBC_.mojo:137-175).
+// D3Q19 / D3Q27 have no reference code; the same construction is used there:
+// s_q = w_q / (2 cs^4) * sum_ab (c_a c_b - cs^2 delta_ab) dev(Pi)_ab,
+// which reproduces the D2Q9 table term by term.
+template <class L, class R>
+__device__ __forceinline__ void kbc_shear(const R (&pi)[6], R (&s)[L::Q]) {
+    if (L::D == 2) {
+        const R N = pi[0] - pi[2];
+        s[0] = 0;
+        s[1] = N * (R)0.25;
+        s[2] = N * (R)0.25;
+        s[3] = -N * (R)0.25;
+        s[4] = -N * (R)0.25;
+        s[5] = pi[1] * (R)0.25;
+        s[6] = pi[1] * (R)0.25;
+        s[7] = -pi[1] * (R)0.25;
+        s[8] = -pi[1] * (R)0.25;
+        return;
+    }
+    const R tr3 = (pi[0] + pi[3] + pi[5]) / (R)3;
+    const R dev[3] = {pi[0] - tr3, pi[3] - tr3, pi[5] - tr3};
+    const R off[3] = {pi[1], pi[2], pi[4]};  // xy, xz, yz
+#pragma unroll
+    for (int q = 0; q < L::Q; ++q) {
+        R diag = 0, offs = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (L::c(q, d) != 0) diag = diag + dev[d];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+            const int a = n == 2 ? 1 : 0, b = n == 0 ? 1 : 2;
+            const int cc = L::c(q, a) * L::c(q, b);
+            if (cc != 0) offs = offs + (cc > 0 ? off[n] : -off[n]);
+        }
+        s[q] = weight<L, R>(q) * (R)4.5 * (diag + (R)2 * offs);
+    }
+}
+
+enum Coll { C_SRT = 0, C_KBC = 2 };
+
+// collide, src/lbm/kernels/steps/collide.mojo:23-90 (SRT: ops/collisions/collision.mojo:12-46,
+// KBC: ops/collisions/KBC_.mojo:180-238).
+template <class L, class R, int COLL, bool LES, bool DDF>
+// inv_tau_base is 1/tau rounded in R (computed once on the host; equal to the
+// per-node (R)1 / tau the reference evaluates when no LES term changes tau).
+__device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R cs) {
+    constexpr int Q = L::Q;
+    const R rho = density<L, R, DDF>(f);
+    R u[3];
+    velocity<L, R>(f, rho, u);
+    const R uu = dot_uu<L, R>(u);
+    R tau_local = tau;
+    if (LES || COLL == C_KBC) {
+        R fn[Q];
+        R pi[6];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fn[q] = f[q] - feq<L, R, DDF>(q, rho, u, uu);
+        pi_neq<L, R>(fn, pi);
+        if (LES) tau_local = tau_local + smagorinsky_tau<L, R>(pi, rho, tau, cs);
+        if (COLL == C_KBC) {
+            const R beta = (R)1 / ((R)2 * tau_local);
+            const R inv_beta = (R)2 * tau_local;
+            R ds[Q];
+            kbc_shear<L, R>(pi, ds);
+            R sp1 = 0, sp2 = 0;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const R fe = feq<L, R, false>(q, rho, u, uu);
+                const R dh = fn[q] - ds[q];
+                sp1 = sp1 + ds[q] * dh / fe;
+                sp2 = sp2 + dh * dh / fe;
+            }
+            const R gamma = inv_beta - ((R)2 - inv_beta) * sp1 / (sp2 + (R)1e-32);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const R dh = fn[q] - ds[q];
+                f[q] = f[q] - beta * (ds[q] * (R)2 + gamma * dh);
+            }
+            return;
+        }
+    }
+    const R inv_tau = LES ? (R)1 / tau_local : inv_tau_base;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = f[q] - inv_tau * (f[q] - feq<L, R, DDF>(q, rho, u, uu));
+}
+
+}  // namespace lbm

@@ -156,4 +156,4 @@ def test_macroscopic_and_reduction(gpu, lat, method):
         assert np.array_equal(u, ur)
     tot = run.reduce(after_odd)
     tref = O.reduce_moments(desc, ref_f, bc, flags, after_odd)
-    np.testing.assert_allclose(tot, tref, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(tot, tref, rtol=1e-12, atol=1e-10)

@@ -140,6 +140,91 @@ def cpu_baseline(desc, tau, budget_s=12.0):
     }
 
 
+def accuracy_study(N, checkpoints, u_lid=0.1, Re=1000.0):
+    """Float16C + DDF (double buffer and esoteric pull) and FP32 runs of the D3Q19 lid-driven cube
+    against FP64 runs of the same case and method: velocity relative L2 and max error at each
+    checkpoint, and the relative drift of the FP64-reduced total mass."""
+    import numpy as np
+    import torch
+
+    from mojo_lbm_amd import _lib
+    from mojo_lbm_amd._lib import LbmDesc
+
+    tau = 0.5 + 3.0 * u_lid * N / Re
+    L = _lib.lib()
+
+    def desc_for(storage, method, ddf):
+        return LbmDesc(D=3, Q=19, nx=N, ny=N, nz=N, tx=8, ty=8, tz=8, method=method, collision=0,
+                       storage=storage, ddf_shift=ddf, les=0, include_moving_boundary=1, included_bcs=3,
+                       smagorinsky_cs=0.1)
+
+    def trajectory(desc):
+        run = make_run(desc, u_lid, True)
+        st, h = run["st"], run["h"]
+        dt = torch.float64 if desc.storage == 2 else torch.float32
+        rho = torch.empty(N ** 3, dtype=dt, device="cuda")
+        u = torch.empty(N ** 3 * 3, dtype=dt, device="cuda")
+        tot = torch.zeros(4, dtype=torch.float64, device="cuda")
+        bufs = [run["f"], run["f2"]]
+        out, k = {}, 0
+
+        def cur():
+            return bufs[k % 2] if desc.method == 0 else bufs[0]
+
+        def moments():
+            _lib.check(L.lbm_macroscopic(h.ptr, rho.data_ptr(), u.data_ptr(), cur().data_ptr(), run["bc"].data_ptr(),
+                                         run["flags"].data_ptr(), int(k % 2 == 0), 0, st))
+            _lib.check(L.lbm_reduce_moments_f64(h.ptr, tot.data_ptr(), cur().data_ptr(), run["bc"].data_ptr(),
+                                                run["flags"].data_ptr(), int(k % 2 == 0), st))
+            torch.cuda.synchronize()
+            return u.double().cpu().numpy().reshape(N, N, N, 3), float(tot[0].item())
+
+        _, m0 = moments()
+        for target in checkpoints:
+            while k < target:
+                if desc.method == 0:
+                    _lib.check(L.lbm_step_db(h.ptr, bufs[(k + 1) % 2].data_ptr(), bufs[k % 2].data_ptr(),
+                                             run["bc"].data_ptr(), run["flags"].data_ptr(), tau, st))
+                else:
+                    _lib.check(L.lbm_step_ep(h.ptr, bufs[0].data_ptr(), run["bc"].data_ptr(), run["flags"].data_ptr(),
+                                             tau, int(k % 2 == 0), st))
+                k += 1
+            uu, m = moments()
+            out[target] = (uu, (m - m0) / m0)
+            print(f"# accuracy: storage={desc.storage} method={desc.method} step {target} mass drift {(m - m0) / m0:.3e}",
+                  file=sys.stderr, flush=True)
+        del run
+        torch.cuda.empty_cache()
+        return out
+
+    fluid = np.zeros((N, N, N), bool)
+    fluid[1:-1, 1:-1, 1:-1] = True
+    results = {}
+    for method, mname in ((0, "double_buffer"), (1, "esoteric_pull")):
+        ref = trajectory(desc_for(2, method, 0))
+        cases = [("f16c_ddf", desc_for(1, method, 1))] + ([("fp32", desc_for(0, method, 0))] if method == 0 else [])
+        for cname, d in cases:
+            tr = trajectory(d)
+            rows = {}
+            for t in checkpoints:
+                u64, drift64 = ref[t]
+                u, drift = tr[t]
+                e = (u - u64)[fluid]
+                rows[str(t)] = {
+                    "u_rel_l2": float(np.linalg.norm(e) / np.linalg.norm(u64[fluid])),
+                    "u_max_abs": float(np.max(np.abs(e))),
+                    "u_max_ref": float(np.max(np.abs(u64[fluid]))),
+                    "mass_drift": drift,
+                    "mass_drift_fp64_run": drift64,
+                }
+            results[f"{cname}_{mname}"] = rows
+            del tr
+        del ref
+    line = {"study": f"D3Q19 lid-driven cube {N}^3, u_lid={u_lid}, Re={Re:g}, tau={tau:.4f}: error vs the FP64 run "
+                     "of the same case and method (fluid nodes)", "results": results}
+    print(json.dumps(line), flush=True)
+
+
 def load_traffic(name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a profile summary
     for this kernel and workload is committed under profiles/."""
@@ -163,7 +248,12 @@ def main():
     ap.add_argument("--xcd-remap", type=int, default=None, help="override LBM_OPT_XCD_REMAP (0 or 1)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain 1 GiB -> 1 GiB device copy kernel (for scaling rocprofv3 byte counters)")
+    ap.add_argument("--accuracy", default=None,
+                    help="comma-separated checkpoints: run the Float16C / FP32 vs FP64 accuracy study instead")
     args = ap.parse_args()
+    if args.accuracy:
+        accuracy_study(args.n, [int(s) for s in args.accuracy.split(",")])
+        return
     if args.calibrate:
         import torch
 

@@ -170,6 +170,28 @@ int lbm_set_exterior_wall(lbm_handle* h, uint8_t* flags, void* bc, const char* s
                           int flag, const double* u, double rho, void* stream);
 
 /*
+ * Interpolated (linear Bouzidi) bounce-back links for lbm_step_db: replaces the
+ * link setup of RigidStationaryObject / get_rigid_sphere and the Bouzidi update of
+ * nodewise_bounceback_kernel (src/lbm/geometry/rigidSphere.mojo:14-170,
+ * src/lbm/geometry/nodewise_interpolated_BB.mojo:46-186), which the reference
+ * leaves without a launcher. Links live on FLUID_SOLID_BOUNDARY nodes; record r
+ * is the node's flag-layout index nodes[r] (strictly increasing) and, for
+ * q = 1..Q-1, qdist[r*(Q-1) + q-1] = the fraction of the link from the node
+ * towards its SOLID pull neighbour x - c_q that lies in the fluid (negative: plain
+ * halfway bounce-back). Host arrays, copied into the handle (n_links = 0 clears).
+ * lbm_macroscopic and lbm_reduce_moments_f64 apply the same links when re-streaming.
+ */
+int lbm_set_links(lbm_handle* h, const uint32_t* nodes, const float* qdist, uint32_t n_links);
+
+/*
+ * Momentum-exchange force on the solid from the interpolated links of the last
+ * lbm_step_db: out_dev[0..D-1] doubles in device memory, summed in a fixed order.
+ * Replaces calculate_drag_around_object / RigidStationaryObject.sum_force
+ * (src/lbm/output/drag.mojo:16-105, src/lbm/geometry/rigidstationary.mojo:155-169).
+ */
+int lbm_link_force(lbm_handle* h, double* out_dev, void* stream);
+
+/*
  * Float16C codec on the device for n values: to_f16c = 1 converts floats to
  * uint16 codes, 0 converts codes to floats. The same codec the F16C storage
  * uses (Float16C.to_fp16c / to_fp32, src/utils/custom_fp.mojo:36-100), for

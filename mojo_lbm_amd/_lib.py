@@ -58,6 +58,8 @@ SIGNATURES = {
     "lbm_reduce_moments_f64": ([_vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
     "lbm_init_equilibrium": ([_vp, _vp, _d, C.POINTER(_d), _vp], _i),
     "lbm_set_exterior_wall": ([_vp, _vp, _vp, _cp, _i, C.POINTER(_d), _d, _vp], _i),
+    "lbm_set_links": ([_vp, _vp, _vp, C.c_uint32], _i),
+    "lbm_link_force": ([_vp, _vp, _vp], _i),
     "lbm_f16c_convert": ([_vp, _vp, _sz, _i, _vp], _i),
     "lbm_set_option": ([_vp, _i, _i], _i),
     "lbm_alloc": ([_sz, C.POINTER(_vp)], _i),

@@ -183,6 +183,9 @@ int lbm_destroy(lbm_handle* hh) {
     Handle* h = H(hh);
     if (h->partial) (void)hipFree(h->partial);
     if (h->summary) (void)hipFree(h->summary);
+    if (h->link_nodes) (void)hipFree(h->link_nodes);
+    if (h->link_q) (void)hipFree(h->link_q);
+    if (h->force_partial) (void)hipFree(h->force_partial);
     delete h;
     return LBM_OK;
 }
@@ -195,9 +198,11 @@ int lbm_step_db(lbm_handle* hh, void* f_out, const void* f_in, const void* bc, c
     if (!f_out || !f_in || !flags || !bc) return set_error(LBM_ERR_INVALID, "NULL buffer");
     if (f_out == f_in) return set_error(LBM_ERR_INVALID, "f_out and f_in must be different buffers");
     if (!(tau > 0.5)) return set_error(LBM_ERR_INVALID, "tau must be > 0.5");
-    return dispatch(h.d, [&](auto l) {
+    const int rc = dispatch(h.d, [&](auto l) {
         return decltype(l)::step(h, f_out, f_in, bc, flags, tau, MODE_DB, (hipStream_t)stream);
     });
+    if (rc == LBM_OK && h.n_links) h.force_valid = true;
+    return rc;
 }
 
 int lbm_step_ep(lbm_handle* hh, void* f, const void* bc, const uint8_t* flags, double tau, int is_even,
@@ -293,6 +298,49 @@ int lbm_set_exterior_wall(lbm_handle* hh, uint8_t* flags, void* bc, const char* 
         hipLaunchKernelGGL((wall_kernel<float>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, flags, (float*)bc, h.g, D,
                            axis, plus, (uint8_t)flag, (float)uu[0], (float)uu[1], (float)uu[2], (float)rho);
     return check_launch("wall_kernel");
+}
+
+int lbm_set_links(lbm_handle* hh, const uint32_t* nodes, const float* qdist, uint32_t n_links) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (h.d.method != LBM_DOUBLE_BUFFER)
+        return set_error(LBM_ERR_UNSUPPORTED, "interpolated bounce-back links need the double-buffer method");
+    if (n_links && (!nodes || !qdist)) return set_error(LBM_ERR_INVALID, "NULL link arrays");
+    const uint32_t n_cells = (uint32_t)((size_t)h.g.n_tiles * h.g.tile_cells);
+    for (uint32_t i = 0; i < n_links; ++i) {
+        if (nodes[i] >= n_cells) return set_error(LBM_ERR_INVALID, "link node index outside the grid");
+        if (i && nodes[i] <= nodes[i - 1]) return set_error(LBM_ERR_INVALID, "link nodes must be strictly increasing");
+    }
+    if (hipDeviceSynchronize() != hipSuccess) return set_error(LBM_ERR_HIP, "synchronize before replacing links failed");
+    if (h.link_nodes) (void)hipFree(h.link_nodes);
+    if (h.link_q) (void)hipFree(h.link_q);
+    h.link_nodes = nullptr;
+    h.link_q = nullptr;
+    h.n_links = 0;
+    h.force_valid = false;
+    if (!n_links) return LBM_OK;
+    const size_t qbytes = (size_t)n_links * (h.d.Q - 1) * sizeof(float);
+    if (hipMalloc((void**)&h.link_nodes, (size_t)n_links * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&h.link_q, qbytes) != hipSuccess)
+        return set_error(LBM_ERR_HIP, "hipMalloc of the link records failed");
+    if (!h.force_partial) {
+        const size_t fb = (size_t)h.g.n_tiles * ((h.g.tile_cells + 63) / 64) * h.d.D * sizeof(double);
+        if (hipMalloc((void**)&h.force_partial, fb) != hipSuccess)
+            return set_error(LBM_ERR_HIP, "hipMalloc of the force partials failed");
+    }
+    if (hipMemcpy(h.link_nodes, nodes, (size_t)n_links * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h.link_q, qdist, qbytes, hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(LBM_ERR_HIP, "copy of the link records failed");
+    h.n_links = n_links;
+    return LBM_OK;
+}
+
+int lbm_link_force(lbm_handle* hh, double* out_dev, void* stream) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (!out_dev) return set_error(LBM_ERR_INVALID, "NULL buffer");
+    if (!h.n_links || !h.force_valid) return set_error(LBM_ERR_INVALID, "no step with links has run");
+    return dispatch(h.d, [&](auto l) { return decltype(l)::link_force(h, out_dev, (hipStream_t)stream); });
 }
 
 int lbm_f16c_convert(const void* in, void* out, size_t n, int to_f16c, void* stream) {

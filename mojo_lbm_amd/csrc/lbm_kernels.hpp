@@ -34,6 +34,10 @@ template <class R> struct StepParams {
     int moving;                // include_moving_boundary
     int eqbc;                  // EQUILIBRIUM in the included BCs
     uint32_t f_bytes;          // bytes of one f buffer (32-bit addressing only)
+    const uint32_t* link_nodes;// interpolated bounce-back records (sorted flag-layout node indices), or NULL
+    const float* link_q;       // (Q-1) link distances per record
+    uint32_t n_links;
+    double* force_partial;     // per wave: D doubles of momentum-exchange force (when link_nodes != NULL)
     Geo g;
 };
 
@@ -44,6 +48,9 @@ template <class R> struct MacroParams {
     R* rho;
     R* u;
     double* partial;           // reduction mode: per-tile (D+1) sums
+    const uint32_t* link_nodes;  // interpolated bounce-back records (double buffer), or null
+    const float* link_q;
+    uint32_t n_links;
     int u_layout;
     int moving;
     int eqbc;
@@ -361,26 +368,88 @@ __device__ __forceinline__ void store_populations(const TileCtx<L>& t, const Fie
     }
 }
 
-// Stream, part 2 (boundary tiles): halfway bounce-back replaces the pull from a
-// SOLID neighbour (raw flag compare, ops/load_and_store.mojo:216) by
-// f(x, opp q), loaded only for those directions; then moving_wall_bc
+// Stream, part 2 (boundary tiles) is halfway_bounce_back, then bouzidi_links on
+// nodes with interpolated links, then apply_boundaries: moving_wall_bc
 // (ops/boundary_condition.mojo:17-83) and equilibrium_bc (:87-204).
+//
+// Halfway bounce-back of the double-buffer pull: the pull from a SOLID
+// neighbour (raw flag compare, ops/load_and_store.mojo:216) is replaced by
+// f(x, opp q), loaded only for those directions.
+template <class L, class S, bool BIG>
+__device__ __forceinline__ void halfway_bounce_back(const TileCtx<L>& t, const uint8_t* sfl, const Field<S, BIG>& f,
+                                                    typename S::R (&fv)[L::Q]) {
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q)
+        if (t.pull_flag(sfl, q) == LBM_FLAG_SOLID) fv[q] = f.ld(t.fbase, (uint32_t)opp(q) * t.tc);
+}
+
+// Interpolated bounce-back (linear Bouzidi, Firdaouss & Lallemand 2001; the formulas of
+// linkwise_bounceback_kernel, src/lbm/geometry/interpolated_BB.mojo:119-140, and
+// nodewise_bounceback_kernel, src/lbm/geometry/nodewise_interpolated_BB.mojo:154-178) on a
+// FLUID_SOLID_BOUNDARY node that has a link record: for each direction whose pull neighbour is
+// SOLID and whose link distance qd >= 0, the population entering along c_q is rebuilt from
+// a = f*(x, opp q) (already in fv[q] by halfway bounce-back) and
+//   qd <= 1/2: b = f*(x + c_q, opp q), f_q = (2 qd) a + (1 - 2 qd) b  (a when x + c_q is SOLID),
+//   qd >  1/2: b = f*(x, q),           f_q = (0.5 / qd) a + ((2 qd - 1) / (2 qd)) b;
+// the link's momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF) is added to force.
+// Returns the rebuilt directions (they take no moving-wall term).
+template <class L, class S, bool DDF, bool BIG>
+__device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uint8_t* sfl, const Field<S, BIG>& f,
+                                                  const uint32_t* __restrict__ link_nodes,
+                                                  const float* __restrict__ link_q, uint32_t n_links, uint32_t node,
+                                                  typename S::R (&fv)[L::Q], double (&force)[3]) {
+    using R = typename S::R;
+    constexpr int Q = L::Q, D = L::D;
+    uint32_t lo = 0, hi = n_links;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (link_nodes[mid] < node) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo >= n_links || link_nodes[lo] != node) return 0;
+    uint32_t done = 0;
+#pragma unroll
+    for (int q = 1; q < Q; ++q) {
+        const R qd = (R)link_q[(size_t)lo * (Q - 1) + (q - 1)];
+        if (!(qd >= (R)0) || t.pull_flag(sfl, q) != LBM_FLAG_SOLID) continue;
+        const R a = fv[q];
+        R fb;
+        if (qd <= (R)0.5) {
+            if ((t.pull_flag(sfl, opp(q)) & 0x0F) == LBM_FLAG_SOLID) {
+                fb = a;
+            } else {
+                const R b = f.ld(t.nb(q, +1), (uint32_t)opp(q) * t.tc);
+                fb = (R)2 * qd * a + ((R)1 - (R)2 * qd) * b;
+            }
+        } else {
+            const R b = f.ld(t.fbase, (uint32_t)q * t.tc);
+            fb = (R)0.5 / qd * a + ((R)2 * qd - (R)1) / ((R)2 * qd) * b;
+        }
+        fv[q] = fb;
+        done |= 1u << q;
+        R s = a + fb;
+        if (DDF) s = s + (R)2 * weight<L, R>(q);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int c = L::c(q, d);
+            if (c != 0) force[d] += (double)(c > 0 ? -s : s);
+        }
+    }
+    return done;
+}
+
 template <class L, class S, int MODE, bool DDF, bool BIG>
 __device__ __forceinline__ void apply_boundaries(const TileCtx<L>& t, const Geo& g, const uint8_t* sfl,
                                                  const Field<S, BIG>& f, const typename S::R* __restrict__ bc,
-                                                 int moving, int eqbc, typename S::R (&fv)[L::Q]) {
+                                                 int moving, int eqbc, typename S::R (&fv)[L::Q],
+                                                 uint32_t rebuilt = 0) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
-    if (MODE == MODE_DB) {
-#pragma unroll
-        for (int q = 1; q < Q; ++q)
-            if (t.pull_flag(sfl, q) == LBM_FLAG_SOLID) fv[q] = f.ld(t.fbase, (uint32_t)opp(q) * t.tc);
-    }
     if (moving) {
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
             const uint8_t nf = t.pull_flag(sfl, q);
-            if ((nf & 0x0F) == LBM_FLAG_SOLID) {
+            if ((nf & 0x0F) == LBM_FLAG_SOLID && !((rebuilt >> q) & 1u)) {
                 R uw[3];
 #pragma unroll
                 for (int d = 0; d < D; ++d) uw[d] = bc[t.pull_index(g, q, D + 1, d)];
@@ -505,15 +574,40 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParam
     if (boundary) t.fetch_flags(p.g, p.flags);
     R fv[Q];
     load_populations<L, S, MODE, BIG>(t, fin, fv);
+    const bool links = MODE == MODE_DB && p.link_nodes != nullptr;  // uniform
+    double force[3] = {0.0, 0.0, 0.0};
+    bool active = true;
     if (boundary) {
         t.commit_flags(p.g, sfl);
         if (t.nonfluid) {
-            if (!node_valid(t.own)) return;
-            apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, fin, p.bc, moving, p.eqbc, fv);
+            if (!node_valid(t.own)) {
+                if (!links) return;
+                active = false;  // stays for the force reduction below
+            } else {
+                if (MODE == MODE_DB) halfway_bounce_back<L, S, BIG>(t, sfl, fin, fv);
+                uint32_t rebuilt = 0;
+                if (links && (t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY)
+                    rebuilt = bouzidi_links<L, S, DDF, BIG>(t, sfl, fin, p.link_nodes, p.link_q, p.n_links,
+                                                            t.tile * t.tc + (uint32_t)t.tid, fv, force);
+                apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, fin, p.bc, moving, p.eqbc, fv, rebuilt);
+            }
         }
     }
-    collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs);
-    store_populations<L, S, MODE, BIG>(t, fout, fv);
+    if (active) {
+        collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs);
+        store_populations<L, S, MODE, BIG>(t, fout, fv);
+    }
+    if (links) {
+        // Momentum-exchange force of this workgroup: wave sums in lane order, one record per wave.
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int waves = (blockDim.x + 63) >> 6;
+#pragma unroll
+        for (int d = 0; d < L::D; ++d) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) force[d] += __shfl_down(force[d], off, 64);
+            if (lane == 0) p.force_partial[((size_t)t.tile * waves + wave) * L::D + d] = force[d];
+        }
+    }
 }
 
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125) and the FP64 moment
@@ -531,7 +625,18 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroPar
     R fv[Q];
     load_populations<L, S, MODE, BIG>(t, f, fv);
     t.commit_flags(p.g, sfl);
-    if (t.nonfluid) apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, f, p.bc, p.moving, p.eqbc, fv);
+    if (t.nonfluid) {
+        uint32_t rebuilt = 0;
+        if (MODE == MODE_DB) {
+            halfway_bounce_back<L, S, BIG>(t, sfl, f, fv);
+            if (p.link_nodes && (t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY) {
+                double unused[3] = {0, 0, 0};
+                rebuilt = bouzidi_links<L, S, DDF, BIG>(t, sfl, f, p.link_nodes, p.link_q, p.n_links,
+                                                        t.tile * t.tc + (uint32_t)t.tid, fv, unused);
+            }
+        }
+        apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
+    }
     R rho = 0, u[3] = {0, 0, 0};
     bool contributes = false;
     if (REDUCE) {

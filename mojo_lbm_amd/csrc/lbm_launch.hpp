@@ -18,7 +18,12 @@ struct Handle {
     size_t f_bytes = 0;          // one population buffer
     bool big = false;            // f_bytes >= 2^31: 64-bit addressing
     double* partial = nullptr;   // per-tile reduction partials (n_tiles * (D+1))
-    uint32_t* summary = nullptr; // per-tile flag OR, rebuilt before every step
+    uint32_t* summary = nullptr; // per-tile region masks, rebuilt before every step
+    uint32_t* link_nodes = nullptr;  // interpolated bounce-back records (handle-owned copies)
+    float* link_q = nullptr;
+    uint32_t n_links = 0;
+    double* force_partial = nullptr; // per wave of every tile: D doubles
+    bool force_valid = false;        // a step with links has run since lbm_set_links
 };
 
 int set_error(int code, const std::string& msg);
@@ -73,6 +78,10 @@ template <class L, class S> struct Launch {
         p.moving = h.d.include_moving_boundary != 0;
         p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
         p.f_bytes = (uint32_t)(h.big ? 0 : h.f_bytes);
+        p.link_nodes = h.n_links ? h.link_nodes : nullptr;
+        p.link_q = h.link_q;
+        p.n_links = h.n_links;
+        p.force_partial = h.force_partial;
         p.g = h.g;
         if (mode == MODE_DB) return step_mode<MODE_DB>(h, p, st);
         if (mode == MODE_EP_EVEN) return step_mode<MODE_EP_EVEN>(h, p, st);
@@ -99,7 +108,8 @@ template <class L, class S> struct Launch {
         return ddf ? macro_k<MODE_EP_ODD, true, REDUCE>(h, p, st) : macro_k<MODE_EP_ODD, false, REDUCE>(h, p, st);
     }
 
-    static MacroParams<R> macro_params(const Handle& h, const void* f, const void* bc, const uint8_t* flags) {
+    static MacroParams<R> macro_params(const Handle& h, const void* f, const void* bc, const uint8_t* flags,
+                                       bool mode_is_db) {
         MacroParams<R> p{};
         p.f = f;
         p.bc = (const R*)bc;
@@ -108,12 +118,17 @@ template <class L, class S> struct Launch {
         p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
         p.f_bytes = (uint32_t)(h.big ? 0 : h.f_bytes);
         p.g = h.g;
+        if (mode_is_db && h.n_links) {
+            p.link_nodes = h.link_nodes;
+            p.link_q = h.link_q;
+            p.n_links = h.n_links;
+        }
         return p;
     }
 
     static int macro(const Handle& h, void* rho, void* u, const void* f, const void* bc, const uint8_t* flags,
                      int mode, int u_layout, hipStream_t st) {
-        MacroParams<R> p = macro_params(h, f, bc, flags);
+        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
         p.rho = (R*)rho;
         p.u = (R*)u;
         p.u_layout = u_layout;
@@ -122,12 +137,20 @@ template <class L, class S> struct Launch {
 
     static int reduce(const Handle& h, double* out, const void* f, const void* bc, const uint8_t* flags, int mode,
                       hipStream_t st) {
-        MacroParams<R> p = macro_params(h, f, bc, flags);
+        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
         p.partial = h.partial;
         int rc = macro_any<true>(h, p, mode, st);
         if (rc) return rc;
         if (L::D == 2) hipLaunchKernelGGL((reduce_finish<3>), dim3(1), dim3(256), 0, st, h.partial, h.g.n_tiles, out);
         else hipLaunchKernelGGL((reduce_finish<4>), dim3(1), dim3(256), 0, st, h.partial, h.g.n_tiles, out);
+        return check_launch("reduce_finish");
+    }
+
+    static int link_force(const Handle& h, double* out, hipStream_t st) {
+        const int waves = (h.g.tile_cells + 63) / 64;
+        const int n = h.g.n_tiles * waves;
+        if (L::D == 2) hipLaunchKernelGGL((reduce_finish<2>), dim3(1), dim3(256), 0, st, h.force_partial, n, out);
+        else hipLaunchKernelGGL((reduce_finish<3>), dim3(1), dim3(256), 0, st, h.force_partial, n, out);
         return check_launch("reduce_finish");
     }
 

@@ -378,10 +378,68 @@ template <class S, class T> struct Stepper {
         }
     }
 
+    /* Interpolated bounce-back links (linear Bouzidi, Firdaouss & Lallemand 2001; the formulas of
+     * linkwise_bounceback_kernel, src/lbm/geometry/interpolated_BB.mojo:119-140, and
+     * nodewise_bounceback_kernel, src/lbm/geometry/nodewise_interpolated_BB.mojo:154-178). Links live on
+     * FLUID_SOLID_BOUNDARY nodes: record r holds the node's flag-layout index link_nodes[r] (sorted) and,
+     * for q = 1..Q-1, link_q[r*(Q-1) + q-1] = the fraction of the link from x towards the SOLID pull
+     * neighbour x - c_q that lies in the fluid (negative: plain halfway bounce-back). The population
+     * entering x along c_q is rebuilt from a = f*(x, opp q) (the one that hit the wall) and
+     *   qd <= 1/2: b = f*(x + c_q, opp q), f_q = (2 qd) a + (1 - 2 qd) b   (halfway bounce-back when
+     *              x + c_q is SOLID),
+     *   qd >  1/2: b = f*(x, q),           f_q = (0.5 / qd) a + ((2 qd - 1) / (2 qd)) b,
+     * and the link adds the momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF shifting) to the
+     * force on the solid. Returns a bit mask of the rebuilt directions (no moving-wall term there). */
+    const uint32_t* link_nodes = nullptr;
+    const float* link_q = nullptr;
+    uint32_t n_links = 0;
+
+    uint32_t bouzidi(T* fv, const uint8_t pf[27], const S* f, const uint8_t* flags, const int idx[3],
+                     double* force) const {
+        if (!n_links || (pf[0] & 0x0F) != LBM_FLAG_FLUID_SOLID_BOUNDARY) return 0;
+        const uint32_t node = (uint32_t)g.addr(idx[0], idx[1], idx[2], 0, 1);
+        uint32_t lo = 0, hi = n_links;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) / 2;
+            if (link_nodes[mid] < node) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo >= n_links || link_nodes[lo] != node) return 0;
+        uint32_t done = 0;
+        for (int q = 1; q < L.Q; ++q) {
+            const T qd = (T)link_q[(size_t)lo * (L.Q - 1) + (q - 1)];
+            if (!(qd >= (T)0) || pf[q] != LBM_FLAG_SOLID) continue;
+            const int oq = L.opp[q];
+            const T a = Store<S, T>::load(f, g.addr(idx[0], idx[1], idx[2], oq, K()));
+            T fb;
+            if (qd <= (T)0.5) {
+                int y[3];
+                adjacent(g, L.D, idx, L.c[q], 1, y);
+                if (flag_is_solid(flags[g.addr(y[0], y[1], y[2], 0, 1)])) {
+                    fb = a;
+                } else {
+                    const T b = Store<S, T>::load(f, g.addr(y[0], y[1], y[2], oq, K()));
+                    fb = (T)2 * qd * a + ((T)1 - (T)2 * qd) * b;
+                }
+            } else {
+                const T b = Store<S, T>::load(f, g.addr(idx[0], idx[1], idx[2], q, K()));
+                fb = (T)0.5 / qd * a + ((T)2 * qd - (T)1) / ((T)2 * qd) * b;
+            }
+            fv[q] = fb;
+            done |= 1u << q;
+            T s = a + fb;
+            if (d.ddf_shift) s = s + (T)2 * cell.w[q];
+            for (int dd = 0; dd < L.D; ++dd)
+                if (L.c[q][dd]) force[dd] += (double)(L.c[q][dd] > 0 ? -s : s);
+        }
+        return done;
+    }
+
     /* moving_wall_bc, src/lbm/kernels/ops/boundary_condition.mojo:17-83:
      * f_q += 2*3*w_q*rho_w*(c_q . u_w) for q >= 1 whose pull neighbour is SOLID (masked enum). */
-    void moving_wall(T* fv, const uint8_t pf[27], const T* bc, const int idx[3]) const {
+    void moving_wall(T* fv, const uint8_t pf[27], const T* bc, const int idx[3], uint32_t skip = 0) const {
         for (int q = 1; q < L.Q; ++q) {
+            if ((skip >> q) & 1u) continue;
             if (!flag_is_solid(pf[q])) continue;
             int n[3];
             adjacent(g, L.D, idx, L.c[q], -1, n);
@@ -467,17 +525,20 @@ template <class S, class T> struct Stepper {
 
     /* stream + apply_boundary_conditions for one node; returns false for invalid nodes. */
     void gather(const S* f, const T* bc, const uint8_t* flags, const int idx[3], int mode, T* fv,
-                uint8_t pf[27]) const {
+                uint8_t pf[27], double* force = nullptr) const {
         // mode: 0 double buffer, 1 esoteric even, 2 esoteric odd
+        uint32_t rebuilt = 0;
         if (mode == 0) {
             gather_flags(flags, idx, pf);
             load_db(f, pf, idx, fv);
+            double fdummy[3] = {0, 0, 0};
+            rebuilt = bouzidi(fv, pf, f, flags, idx, force ? force : fdummy);
         } else {
             load_ep(f, idx, mode == 1, fv);
             pf[0] = flags[g.addr(idx[0], idx[1], idx[2], 0, 1)];
             if (d.include_moving_boundary) gather_flags(flags, idx, pf);
         }
-        if (d.include_moving_boundary) moving_wall(fv, pf, bc, idx);
+        if (d.include_moving_boundary) moving_wall(fv, pf, bc, idx, rebuilt);
         if (eq_enabled()) equilibrium(fv, pf, bc, idx);
     }
 };
@@ -486,11 +547,16 @@ int g_threads = 1;
 
 template <class S, class T>
 int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uint8_t* flags, double tau,
-              int mode) {
+              int mode, const uint32_t* link_nodes = nullptr, const float* link_q = nullptr, uint32_t n_links = 0,
+              double* force_out = nullptr) {
     Stepper<S, T> st(*d);
+    st.link_nodes = link_nodes;
+    st.link_q = link_q;
+    st.n_links = n_links;
     const Geo& g = st.g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const T tau_t = (T)tau;
+    std::vector<double> nforce(force_out ? (size_t)ncell * 3 : 0, 0.0);
 #pragma omp parallel for schedule(static) num_threads(g_threads)
     for (long i = 0; i < ncell; ++i) {
         int idx[3] = {(int)(i % g.nx), (int)((i / g.nx) % g.ny), (int)(i / ((long)g.nx * g.ny))};
@@ -498,7 +564,7 @@ int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uin
         if (!st.valid(flag)) continue;
         T fv[27];
         uint8_t pf[27];
-        st.gather(f_in, bc, flags, idx, mode, fv, pf);
+        st.gather(f_in, bc, flags, idx, mode, fv, pf, force_out ? &nforce[(size_t)i * 3] : nullptr);
         st.collide(fv, tau_t);
         if (mode == 0) {
             for (int q = 0; q < st.L.Q; ++q) Store<S, T>::store(f_out, g.addr(idx[0], idx[1], idx[2], q, st.K()), fv[q]);
@@ -506,14 +572,23 @@ int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uin
             st.store_ep(f_out, idx, mode == 1, fv);
         }
     }
+    if (force_out) {
+        for (int k = 0; k < 3; ++k) force_out[k] = 0.0;
+        for (long i = 0; i < ncell; ++i)
+            for (int k = 0; k < 3; ++k) force_out[k] += nforce[(size_t)i * 3 + k];
+    }
     return 0;
 }
 
 /* calculate_rho_and_velocity, src/lbm/output/velocity.mojo:25-125 (tau = 0, solid nodes report bc). */
 template <class S, class T>
 int macro_impl(const lbm_desc* d, T* rho_out, T* u_out, const S* f, const T* bc, const uint8_t* flags, int mode,
-               int u_layout) {
+               int u_layout, const uint32_t* link_nodes = nullptr, const float* link_q = nullptr,
+               uint32_t n_links = 0) {
     Stepper<S, T> st(*d);
+    st.link_nodes = link_nodes;
+    st.link_q = link_q;
+    st.n_links = n_links;
     const Geo& g = st.g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const int D = st.L.D;
@@ -544,8 +619,12 @@ int macro_impl(const lbm_desc* d, T* rho_out, T* u_out, const S* f, const T* bc,
 
 /* FP64 sums of rho and rho*u over valid nodes, in node order (x fastest). */
 template <class S, class T>
-int reduce_impl(const lbm_desc* d, double* out, const S* f, const T* bc, const uint8_t* flags, int mode) {
+int reduce_impl(const lbm_desc* d, double* out, const S* f, const T* bc, const uint8_t* flags, int mode,
+                const uint32_t* link_nodes = nullptr, const float* link_q = nullptr, uint32_t n_links = 0) {
     Stepper<S, T> st(*d);
+    st.link_nodes = link_nodes;
+    st.link_q = link_q;
+    st.n_links = n_links;
     const Geo& g = st.g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const int D = st.L.D;
@@ -656,6 +735,18 @@ int orc_step_db(const lbm_desc* d, void* f_out, const void* f_in, const void* bc
                  (step_impl<double, double>(d, (double*)f_out, (const double*)f_in, (const double*)bc, flags, tau, 0)))
 }
 
+/* Double-buffer step with interpolated bounce-back links; force_out[0..2] = total momentum-exchange force. */
+int orc_step_db_links(const lbm_desc* d, void* f_out, const void* f_in, const void* bc, const uint8_t* flags,
+                      double tau, const uint32_t* link_nodes, const float* link_q, uint32_t n_links,
+                      double* force_out) {
+    ORC_DISPATCH((step_impl<float, float>(d, (float*)f_out, (const float*)f_in, (const float*)bc, flags, tau, 0,
+                                          link_nodes, link_q, n_links, force_out)),
+                 (step_impl<uint16_t, float>(d, (uint16_t*)f_out, (const uint16_t*)f_in, (const float*)bc, flags, tau,
+                                             0, link_nodes, link_q, n_links, force_out)),
+                 (step_impl<double, double>(d, (double*)f_out, (const double*)f_in, (const double*)bc, flags, tau, 0,
+                                            link_nodes, link_q, n_links, force_out)))
+}
+
 int orc_step_ep(const lbm_desc* d, void* f, const void* bc, const uint8_t* flags, double tau, int is_even) {
     int mode = is_even ? 1 : 2;
     ORC_DISPATCH((step_impl<float, float>(d, (float*)f, (const float*)f, (const float*)bc, flags, tau, mode)),
@@ -677,6 +768,23 @@ int orc_reduce_moments(const lbm_desc* d, double* out, const void* f, const void
     ORC_DISPATCH((reduce_impl<float, float>(d, out, (const float*)f, (const float*)bc, flags, mode)),
                  (reduce_impl<uint16_t, float>(d, out, (const uint16_t*)f, (const float*)bc, flags, mode)),
                  (reduce_impl<double, double>(d, out, (const double*)f, (const double*)bc, flags, mode)))
+}
+
+/* Double-buffer macroscopic fields / FP64 totals on a grid with interpolated bounce-back links. */
+int orc_macroscopic_links(const lbm_desc* d, void* rho, void* u, const void* f, const void* bc, const uint8_t* flags,
+                          int u_layout, const uint32_t* nodes, const float* q, uint32_t n) {
+    if (d->method != LBM_DOUBLE_BUFFER) return -1;
+    ORC_DISPATCH((macro_impl<float, float>(d, (float*)rho, (float*)u, (const float*)f, (const float*)bc, flags, 0, u_layout, nodes, q, n)),
+                 (macro_impl<uint16_t, float>(d, (float*)rho, (float*)u, (const uint16_t*)f, (const float*)bc, flags, 0, u_layout, nodes, q, n)),
+                 (macro_impl<double, double>(d, (double*)rho, (double*)u, (const double*)f, (const double*)bc, flags, 0, u_layout, nodes, q, n)))
+}
+
+int orc_reduce_moments_links(const lbm_desc* d, double* out, const void* f, const void* bc, const uint8_t* flags,
+                             const uint32_t* nodes, const float* q, uint32_t n) {
+    if (d->method != LBM_DOUBLE_BUFFER) return -1;
+    ORC_DISPATCH((reduce_impl<float, float>(d, out, (const float*)f, (const float*)bc, flags, 0, nodes, q, n)),
+                 (reduce_impl<uint16_t, float>(d, out, (const uint16_t*)f, (const float*)bc, flags, 0, nodes, q, n)),
+                 (reduce_impl<double, double>(d, out, (const double*)f, (const double*)bc, flags, 0, nodes, q, n)))
 }
 
 int orc_init_equilibrium(const lbm_desc* d, void* f, double rho, const double* u) {

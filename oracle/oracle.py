@@ -42,6 +42,12 @@ def lib():
         L.orc_addr.restype = C.c_size_t
         L.orc_step_db.argtypes = [pd, vp, vp, vp, vp, d]
         L.orc_step_ep.argtypes = [pd, vp, vp, vp, d, i]
+        L.orc_step_db_links.argtypes = [pd, vp, vp, vp, vp, d, vp, vp, C.c_uint32, vp]
+        L.orc_step_db_links.restype = i
+        L.orc_macroscopic_links.argtypes = [pd, vp, vp, vp, vp, vp, i, vp, vp, C.c_uint32]
+        L.orc_macroscopic_links.restype = i
+        L.orc_reduce_moments_links.argtypes = [pd, vp, vp, vp, vp, vp, vp, C.c_uint32]
+        L.orc_reduce_moments_links.restype = i
         L.orc_macroscopic.argtypes = [pd, vp, vp, vp, vp, vp, i, i]
         L.orc_reduce_moments.argtypes = [pd, vp, vp, vp, vp, i]
         L.orc_init_equilibrium.argtypes = [pd, vp, d, vp]
@@ -114,21 +120,48 @@ def step_db(desc, f_out, f_in, bc, flags, tau):
     _ok(lib().orc_step_db(C.byref(desc), _p(f_out), _p(f_in), _p(bc), _p(flags), float(tau)), "step_db")
 
 
+def step_db_links(desc, f_out, f_in, bc, flags, tau, nodes, qdist):
+    """Double-buffer step with interpolated bounce-back links; returns the momentum-exchange force (D,)."""
+    nodes = np.ascontiguousarray(nodes, np.uint32)
+    qdist = np.ascontiguousarray(qdist, np.float32)
+    force = np.zeros(3, np.float64)
+    _ok(lib().orc_step_db_links(C.byref(desc), _p(f_out), _p(f_in), _p(bc), _p(flags), float(tau), _p(nodes),
+                                _p(qdist), len(nodes), _p(force)), "step_db_links")
+    return force[: desc.D]
+
+
 def step_ep(desc, f, bc, flags, tau, is_even):
     _ok(lib().orc_step_ep(C.byref(desc), _p(f), _p(bc), _p(flags), float(tau), int(bool(is_even))), "step_ep")
 
 
-def macroscopic(desc, f, bc, flags, after_odd=False, u_layout=0):
+def _links(links):
+    nodes = np.ascontiguousarray(links[0], np.uint32)
+    qd = np.ascontiguousarray(links[1], np.float32)
+    return nodes, qd
+
+
+def macroscopic(desc, f, bc, flags, after_odd=False, u_layout=0, links=None):
+    """rho, u after re-streaming; links = (nodes, qdist) applies interpolated bounce-back (double buffer)."""
     n = n_cells(desc)
     rho = np.zeros(n, r_dtype(desc))
     u = np.zeros(n * desc.D, r_dtype(desc))
+    if links is not None:
+        nodes, qd = _links(links)
+        _ok(lib().orc_macroscopic_links(C.byref(desc), _p(rho), _p(u), _p(f), _p(bc), _p(flags), int(u_layout),
+                                        _p(nodes), _p(qd), len(nodes)), "macroscopic_links")
+        return rho, u
     _ok(lib().orc_macroscopic(C.byref(desc), _p(rho), _p(u), _p(f), _p(bc), _p(flags), int(after_odd),
                               int(u_layout)), "macroscopic")
     return rho, u
 
 
-def reduce_moments(desc, f, bc, flags, after_odd=False):
+def reduce_moments(desc, f, bc, flags, after_odd=False, links=None):
     out = np.zeros(desc.D + 1, np.float64)
+    if links is not None:
+        nodes, qd = _links(links)
+        _ok(lib().orc_reduce_moments_links(C.byref(desc), _p(out), _p(f), _p(bc), _p(flags), _p(nodes), _p(qd),
+                                           len(nodes)), "reduce_links")
+        return out
     _ok(lib().orc_reduce_moments(C.byref(desc), _p(out), _p(f), _p(bc), _p(flags), int(after_odd)), "reduce")
     return out
 

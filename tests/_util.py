@@ -115,6 +115,18 @@ class GpuRun:
         t.cuda.synchronize()
         return out.cpu().numpy()
 
+    def set_links(self, nodes, qdist):
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        qdist = np.ascontiguousarray(qdist, np.float32)
+        _lib.check(self.L.lbm_set_links(self.h.ptr, nodes.ctypes.data, qdist.ctypes.data, len(nodes)))
+
+    def link_force(self):
+        t = self.torch
+        out = t.zeros(3, dtype=t.float64, device="cuda")
+        _lib.check(self.L.lbm_link_force(self.h.ptr, out.data_ptr(), self.stream))
+        t.cuda.synchronize()
+        return out.cpu().numpy()[: self.desc.D]
+
     def f_host(self):
         self.torch.cuda.synchronize()
         return self.f.cpu().numpy()
@@ -163,3 +175,35 @@ def db_to_ep_odd(desc, f):
         shifted = np.roll(idx[..., p], shift=(-c[p][0], -c[p][1], -c[p][2]), axis=(0, 1, 2))
         out[shifted] = f[idx[..., p]]
     return out
+
+
+def grid_of(desc):
+    """Host LBM_Grid with the description's lattice, sizes and tile."""
+    import mojo_lbm_amd as lbm
+
+    lat = {9: lbm.get_D2Q9, 19: lbm.get_D3Q19, 27: lbm.get_D3Q27}[desc.Q]()
+    return lbm.LBM_Grid(lat, desc.nx, desc.ny, desc.nz, (desc.tx, desc.ty, desc.tz))
+
+
+def obstacle_case(lat="D2Q9", storage=0, ddf=0, seed=0, amp=1e-3, u_in=0.05, collision=0, les=0):
+    """Channel with an immersed circle (2-D) or sphere (3-D) in the reference cylinder script's
+    boundary order (benchmarks/Cylinder2D/CylinderFlow_Re20.mojo:147-150): +X EQUILIBRIUM outlet,
+    -X SOLID inlet with velocity, -Y/+Y SOLID walls. Seeded noise on every population.
+    Returns desc, f, bc, flags, nodes, qdist."""
+    from mojo_lbm_amd import geometry as G
+
+    if lat == "D2Q9":
+        n, center, radius = (64, 32, 1), (16.3, 15.6), 5.3
+    else:
+        n, center, radius = (32, 16, 16), (10.4, 7.7, 8.1), 4.2
+    desc = make_desc(lat, n, storage=storage, ddf=ddf, eq=True, collision=collision, les=les)
+    f, bc, flags = perturbed_state(desc, seed=seed, amp=amp, walls=False)
+    D = desc.D
+    O.set_exterior_wall(desc, flags, bc, "+X", 2, None, 1.0)
+    O.set_exterior_wall(desc, flags, bc, "-X", 1, [u_in] + [0.0] * (D - 1), 1.0)
+    O.set_exterior_wall(desc, flags, bc, "-Y", 1, [0.0] * D, 1.0)
+    O.set_exterior_wall(desc, flags, bc, "+Y", 1, [0.0] * D, 1.0)
+    grid = grid_of(desc)
+    inside = G.add_sphere(grid, flags, center, radius)
+    nodes, qdist = G.sphere_links(grid, flags, center, radius, inside)
+    return desc, f, bc, flags, nodes, qdist

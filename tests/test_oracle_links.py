@@ -1,0 +1,100 @@
+"""Pins the oracle's interpolated bounce-back (linear Bouzidi) and momentum-exchange force.
+
+The reference's link-wise kernel (src/lbm/geometry/interpolated_BB.mojo:119-140) has no
+golden vectors, so the oracle is pinned by what the mathematics fixes:
+  * steady Couette flow between a stationary wall at an arbitrary link fraction qd and a
+    halfway moving wall: the linear scheme reproduces the linear profile exactly (its
+    interpolation is exact for linear fields and the bounce-back is exact at the wall), and
+    the momentum exchange equals the exact wall shear nu*U/H per unit length in x and the
+    pressure rho*cs^2 per unit length in y. Both branches (qd <= 1/2, qd > 1/2), both storages of
+    the rest value (DDF shift on/off) and several tau are covered; a wrong coefficient,
+    branch, direction or sign breaks the profile or the force by O(U);
+  * qd = 1/2 reduces to halfway bounce-back bit for bit (2*0.5*a + 0*b == a);
+  * the geometry's link fractions lie on the sphere.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import _util as U
+
+
+def _couette(qd, tau, ddf, steps, U_wall=0.05, n=16):
+    desc = U.make_desc("D2Q9", (n, n, 1), storage=2, ddf=ddf)
+    f, bc, flags = O.alloc(desc)
+    O.init_equilibrium(desc, f)
+    O.set_exterior_wall(desc, flags, bc, "+Y", 1, [U_wall, 0.0], 1.0)
+    O.set_exterior_wall(desc, flags, bc, "-Y", 1, [0.0, 0.0], 1.0)
+    c, _, _ = O.lattice(2, 9)
+    nodes = np.array(sorted(O.addr(desc, x, 1, 0, 0, 1) for x in range(n)), np.uint32)
+    qdist = np.full((n, 8), -1.0, np.float32)
+    qdist[:, c[1:, 1] == 1] = qd  # links towards the wall row y = 0
+    flags[nodes] = 3
+    a, b = f.copy(), f.copy()
+    for _ in range(steps):
+        force = O.step_db_links(desc, b, a, bc, flags, tau, nodes, qdist)
+        a, b = b, a
+    rho, u = O.macroscopic(desc, a, bc, flags, links=(nodes, qdist))
+    y = np.arange(n)
+    y_wall = 1.0 - float(np.float32(qd))
+    y_top = n - 1.5
+    exact = U_wall * (y - y_wall) / (y_top - y_wall)
+    ux = u.reshape(n, n, 2)[:, :, 0]
+    nu = (tau - 0.5) / 3.0
+    return ux[:, 1:n - 1], exact[1:n - 1], force, nu * U_wall / (y_top - y_wall) * n, n / 3.0
+
+
+@pytest.mark.parametrize("qd,tau,ddf,steps", [(0.2, 1.0, 0, 4000), (0.5, 1.0, 1, 4000), (0.7, 0.7, 1, 12000),
+                                              (0.95, 1.4, 0, 6000), (0.35, 0.8, 0, 8000)])
+def test_bouzidi_couette_is_exact(qd, tau, ddf, steps):
+    ux, exact, force, fx, fy = _couette(qd, tau, ddf, steps)
+    assert np.max(np.abs(ux - exact[None, :])) < 1e-10 * 0.05
+    assert abs(force[0] - fx) < 1e-9 * fx
+    assert abs(force[1] + fy) < 1e-9 * fy
+
+
+def test_bouzidi_couette_detects_wrong_fraction():
+    # the same run read against a wall at the halfway position fails: the check is sensitive to qd
+    ux, exact, _, _, _ = _couette(0.2, 1.0, 0, 4000)
+    halfway = 0.05 * (np.arange(1, 15) - 0.5) / (14.5 - 0.5)
+    assert np.max(np.abs(ux[0] - halfway)) > 1e-4
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+def test_half_fraction_is_halfway_bounce_back(lat):
+    desc, f, bc, flags, nodes, qdist = U.obstacle_case(lat, seed=3)
+    half = np.where(qdist >= 0, np.float32(0.5), np.float32(-1.0))
+    a = f.copy()
+    b = f.copy()
+    O.step_db(desc, a, f, bc, flags, 0.6)
+    O.step_db_links(desc, b, f, bc, flags, 0.6, nodes, half)
+    assert np.array_equal(a, b)
+    # and a fraction away from 1/2 changes exactly the link nodes' rebuilt populations
+    c = f.copy()
+    O.step_db_links(desc, c, f, bc, flags, 0.6, nodes, qdist)
+    assert not np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19"])
+def test_sphere_link_fractions_lie_on_the_sphere(lat):
+    desc, f, bc, flags, nodes, qdist = U.obstacle_case(lat)
+    grid = U.grid_of(desc)
+    center = (16.3, 15.6, 0.0) if lat == "D2Q9" else (10.4, 7.7, 8.1)
+    radius = 5.3 if lat == "D2Q9" else 4.2
+    assert np.all(np.diff(nodes.astype(np.int64)) > 0)
+    c, _, _ = O.lattice(desc.D, desc.Q)
+    amap = U.index_map(desc, 1)[..., 0]
+    pos = {int(amap[x, y, z]): (x, y, z) for x, y, z in zip(*np.nonzero(np.ones(grid.shape, bool)))}
+    checked = 0
+    for node, qs in zip(nodes, qdist):
+        x = np.array(pos[int(node)], np.float64)
+        assert flags[node] == 3
+        for q in range(1, desc.Q):
+            s = x - c[q]
+            solid = np.sum((s - center) ** 2) <= radius ** 2
+            assert (qs[q - 1] >= 0) == solid
+            if solid:
+                p = x + qs[q - 1] * (s - x)
+                assert abs(np.linalg.norm(p - center) - radius) < 1e-4
+                checked += 1
+    assert checked > 20

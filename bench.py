@@ -225,6 +225,100 @@ def accuracy_study(N, checkpoints, u_lid=0.1, Re=1000.0):
     print(json.dumps(line), flush=True)
 
 
+def cylinder_batch(steps, autotune_steps=100, seed=2024, compare=False):
+    """64 seeded 2-D cylinder channels (D2Q9 SRT FP32, interpolated bounce-back) advanced together:
+    one launch per step for all cases, hipGraph replays. Times `steps` steps from rest after the
+    autotuner has picked the tile shape and grid; returns aggregate MLUPs over the real (unpadded)
+    nodes and the per-case drag / lift coefficients. compare=True also runs every case alone
+    through lbm_step_db and compares its populations and coefficients."""
+    import numpy as np
+    import torch
+
+    from mojo_lbm_amd import DoubleBufferConfig, Flags, get_D2Q9
+    from mojo_lbm_amd.batch import CaseBatch
+    from mojo_lbm_amd.cylinder import random_cases
+
+    lat = get_D2Q9()
+    cases = random_cases(64, seed)
+    b = CaseBatch(lat, [(c.nx, c.ny, 1) for c in cases], [c.tau for c in cases],
+                  DoubleBufferConfig(BCs={Flags.EQUILIBRIUM}))
+    for i, c in enumerate(cases):
+        flags, bc, links = c.arrays(lat)
+        b.set_case(i, flags, bc, links)
+    t0 = time.time()
+    tile, grid = b.autotune(autotune_steps)
+    tune_s = time.time() - t0
+    info = b.info()
+    b.init_equilibrium()
+    b.run(info["graph_steps"])  # captures the graph outside the timed region
+    b.init_equilibrium()
+    f0 = [b.get_f(i).cpu().numpy() for i in range(len(cases))] if compare else None
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    b.run(steps)
+    end.record()
+    torch.cuda.synchronize()
+    t = start.elapsed_time(end) / 1e3
+    forces = b.force()
+    coef = np.array([c.coefficients(f) for c, f in zip(cases, forces)])
+    real = info["real_nodes"]
+    bytes_step = real * 2 * lat.Q * 4 + info["padded_nodes"]
+    out = {
+        "workload": f"64 D2Q9 SRT FP32 cylinder channels (seed {seed}), nx 200..800, interpolated bounce-back, "
+                    f"{steps} steps each from rest, one launch per step for all cases",
+        "cases": len(cases), "real_nodes": real, "padded_nodes": info["padded_nodes"], "steps": steps,
+        "tile": list(tile), "grid_blocks": grid, "graph_steps": info["graph_steps"],
+        "autotune_s": round(tune_s, 2),
+        "seconds": round(t, 4), "ms_per_step": round(t / steps * 1e3, 5),
+        "mlups": round(real * steps / t / 1e6, 1),
+        "roofline_frac": round(bytes_step / (t / steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "tau_range": [round(min(c.tau for c in cases), 5), round(max(c.tau for c in cases), 5)],
+        "unstable_cases": int(np.sum(~np.isfinite(coef).all(axis=1))),
+        "cd": [float(np.nanmin(coef[:, 0])), float(np.nanmedian(coef[:, 0])), float(np.nanmax(coef[:, 0]))],
+        "cl_abs_max": float(np.nanmax(np.abs(coef[:, 1]))),
+    }
+    if compare:
+        from mojo_lbm_amd import _lib
+
+        L = _lib.lib()
+        st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        same, dcd, dcl, t_single = 0, 0.0, 0.0, 0.0
+        per_case = []
+        for i, c in enumerate(cases):
+            d, flags, bc, nodes, qd = b.export_case(i)
+            h = _lib.Handle(d)
+            _lib.check(L.lbm_set_option(h.ptr, _lib.OPT_STATIC_FLAGS, 1))
+            _lib.check(L.lbm_set_links(h.ptr, nodes.ctypes.data, qd.ctypes.data, len(nodes)))
+            fa = torch.from_numpy(f0[i]).cuda()
+            fb = fa.clone()
+            bct = torch.from_numpy(bc).cuda()
+            flt = torch.from_numpy(flags).cuda()
+            torch.cuda.synchronize()
+            t1 = time.time()
+            for _ in range(steps):
+                _lib.check(L.lbm_step_db(h.ptr, fb.data_ptr(), fa.data_ptr(), bct.data_ptr(), flt.data_ptr(),
+                                         c.tau, st))
+                fa, fb = fb, fa
+            torch.cuda.synchronize()
+            t_single += time.time() - t1
+            fo = torch.zeros(3, dtype=torch.float64, device="cuda")
+            _lib.check(L.lbm_link_force(h.ptr, fo.data_ptr(), st))
+            cs = c.coefficients(fo.cpu().numpy())
+            eq = bool(torch.equal(fa.view(torch.uint8), b.get_f(i).view(torch.uint8)))
+            same += eq
+            dcd = max(dcd, abs(cs[0] - coef[i, 0]))
+            dcl = max(dcl, abs(cs[1] - coef[i, 1]))
+            per_case.append({"nx": c.nx, "ny": c.ny, "D": round(c.diameter, 3), "Re": round(c.re, 2),
+                             "tau": round(c.tau, 5), "cd_batch": coef[i, 0], "cl_batch": coef[i, 1],
+                             "cd_single": cs[0], "cl_single": cs[1], "bitwise_equal": eq})
+            del h
+        out["compare"] = {"bitwise_equal_cases": same, "max_abs_dcd": dcd, "max_abs_dcl": dcl,
+                          "single_runs_seconds": round(t_single, 2), "per_case": per_case}
+    b.close()
+    return out
+
+
 def load_traffic(name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a profile summary
     for this kernel and workload is committed under profiles/."""
@@ -250,7 +344,14 @@ def main():
                     help="run only a plain 1 GiB -> 1 GiB device copy kernel (for scaling rocprofv3 byte counters)")
     ap.add_argument("--accuracy", default=None,
                     help="comma-separated checkpoints: run the Float16C / FP32 vs FP64 accuracy study instead")
+    ap.add_argument("--cylinders", type=int, default=20000,
+                    help="steps of the 64-case batched cylinder run reported under 'cylinder_batch' (0: skip)")
+    ap.add_argument("--cylinder-compare", action="store_true",
+                    help="run only the batched cylinder cases, then every case alone, and compare")
     args = ap.parse_args()
+    if args.cylinder_compare:
+        print(json.dumps(cylinder_batch(args.cylinders or 20000, compare=True)), flush=True)
+        return
     if args.accuracy:
         accuracy_study(args.n, [int(s) for s in args.accuracy.split(",")])
         return
@@ -273,6 +374,7 @@ def main():
 
     import torch
 
+    from mojo_lbm_amd import _lib
     from mojo_lbm_amd._lib import LbmDesc
 
     N = args.n
@@ -299,10 +401,10 @@ def main():
         storage, method, ddf, s, walls = variants[name]
         d = desc_for(storage, method, ddf)
         run = make_run(d, u_lid, walls)
+        # flags and bc are static in this benchmark: the flag summary is built once
+        _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_STATIC_FLAGS, 1))
         if args.xcd_remap is not None:
-            from mojo_lbm_amd import _lib as _L
-
-            _L.check(run["L"].lbm_set_option(run["h"].ptr, _L.OPT_XCD_REMAP, args.xcd_remap))
+            _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_XCD_REMAP, args.xcd_remap))
         steps = args.steps if (name == "fp32" or args.only) else max(100, args.steps // 2)
         t = time_run(run, tau, steps, args.warmup)
         B = algorithmic_bytes(19, N, N, N, s, walls)
@@ -339,7 +441,7 @@ def main():
             "workload": f"D3Q19 SRT lid-driven cube {N}^3, double-buffer pull, (x,y,z,q) 8x8x8 tiles, "
                         f"u_lid=0.1, Re=100, tau={tau:.3f}",
             "lattice": "D3Q19", "grid": [N, N, N], "tile": [8, 8, 8], "method": "double_buffer",
-            "storage": head, "parallelism": "single GPU",
+            "storage": head, "parallelism": "single GPU", "static_flags": True,
         },
         "roofline": {
             "bound": "hbm",
@@ -354,6 +456,11 @@ def main():
     }
     if "fp32" in results and "f16c" in results:
         line["variants"]["f16c_over_fp32"] = round(results["f16c"]["mlups"] / results["fp32"]["mlups"], 3)
+    if args.cylinders and not args.only:
+        cyl = cylinder_batch(args.cylinders)
+        print(f"# cylinder batch: {cyl['mlups']:.0f} MLUPs over {cyl['cases']} cases, tile {cyl['tile']}, "
+              f"grid {cyl['grid_blocks']}, {cyl['roofline_frac']:.1%} of 8 TB/s", file=sys.stderr, flush=True)
+        line["cylinder_batch"] = cyl
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(desc_for(0, 0, 0), tau)
     else:

@@ -205,9 +205,14 @@ int lbm_f16c_convert(const void* in, void* out, size_t n, int to_f16c, void* str
  *   LBM_OPT_XCD_REMAP: 1 = give each of the 8 XCDs a contiguous run of tiles
  *                      (default when the tile count is a multiple of 8),
  *                      0 = hardware round-robin order.
+ *   LBM_OPT_STATIC_FLAGS: 1 = the caller promises that flags and bc do not change
+ *                      while the same flags / bc buffers are passed: the per-tile
+ *                      flag summary is rebuilt only when either pointer changes
+ *                      (or this option is set again), not before every step.
+ *                      0 (default) = rebuilt before every step.
  * Returns LBM_ERR_INVALID for an unknown option or value.
  */
-enum { LBM_OPT_XCD_REMAP = 1 };
+enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2 };
 int lbm_set_option(lbm_handle* h, int option, int value);
 
 /* Device memory helpers for hosts without their own GPU runtime (the Mojo host
@@ -218,6 +223,61 @@ int lbm_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes, void* stre
 int lbm_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes, void* stream);
 int lbm_memset(void* dst_dev, int value, size_t bytes, void* stream);
 int lbm_synchronize(void* stream);
+
+/*
+ * Batched cases. Many independent double-buffer grids of different sizes and
+ * relaxation times -- e.g. a sweep of 2-D cylinder channels -- advance together,
+ * one kernel launch per step for all of them, runs of steps replayed from a
+ * captured hipGraph. Drives what the reference does one case per script with
+ * Solver.double_buffer_step (src/lbm/solver.mojo:89-110) and reads the drag of
+ * calculate_drag_around_object (src/lbm/output/drag.mojo:16-105) per case; the
+ * reference has no batched launch. The batch owns its buffers. Each case is
+ * padded to whole tiles with SOLID nodes (a case relying on periodic wrap needs
+ * sizes that are tile multiples). Case arrays are host memory in row-major
+ * (x, y, z) node order (x slowest, as lbm_macroscopic's rho): flags one byte
+ * per node, bc D+1 doubles per node (u..., rho), link records as lbm_set_links
+ * but with row-major node ids. proto gives D, Q, tile, storage, collision, LES,
+ * DDF, BC set (its nx/ny/nz are ignored; method must be double buffer).
+ */
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_create(const lbm_desc* proto, int32_t n_cases, const int32_t* sizes /* n_cases*3 */,
+                     const double* taus, lbm_batch** out);
+int lbm_batch_destroy(lbm_batch* b);
+/* NULL flags: all FLUID; NULL bc: u = 0, rho = 1. Takes effect at the next run / init. */
+int lbm_batch_set_case(lbm_batch* b, int32_t c, const uint8_t* flags, const double* bc,
+                       const uint32_t* link_nodes, const float* qdist, uint32_t n_links);
+/* Re-lays out every case for another tile shape; the state must be initialised again. */
+int lbm_batch_set_tile(lbm_batch* b, int32_t tx, int32_t ty, int32_t tz);
+/*
+ *   LBM_BATCH_OPT_GRID_BLOCKS: 0 = one workgroup per tile (default), n > 0 = a
+ *                              persistent grid of n workgroups (rounded up to a
+ *                              multiple of 8) looping over the tiles.
+ *   LBM_BATCH_OPT_GRAPH_STEPS: steps per captured hipGraph (even; 0 = no graph;
+ *                              default 100).
+ */
+enum { LBM_BATCH_OPT_GRID_BLOCKS = 2, LBM_BATCH_OPT_GRAPH_STEPS = 3 };
+int lbm_batch_set_option(lbm_batch* b, int option, int value);
+/* Both buffers of every case to f_eq(rho = 1, u = 0); resets the step parity. */
+int lbm_batch_init_equilibrium(lbm_batch* b, void* stream);
+int lbm_batch_run(lbm_batch* b, int64_t steps, void* stream);
+/* Momentum-exchange force of the last step per case: out_dev[c*D + d] (device doubles). */
+int lbm_batch_force(lbm_batch* b, double* out_dev, void* stream);
+/* Padded single-grid description of case c and its staged tiled flags / bc (grid float
+ * type) / link records, to rerun the case alone with lbm_create + lbm_set_links. */
+int lbm_batch_case_desc(lbm_batch* b, int32_t c, lbm_desc* out);
+int lbm_batch_export_case(lbm_batch* b, int32_t c, uint8_t* flags, void* bc, uint32_t* link_nodes,
+                          float* qdist, uint32_t* n_links);
+/* Current populations of case c (its tiled layout) to / from device memory; set_f writes both buffers. */
+int lbm_batch_get_f(lbm_batch* b, int32_t c, void* dst_dev, void* stream);
+int lbm_batch_set_f(lbm_batch* b, int32_t c, const void* src_dev, void* stream);
+/*
+ * Times `steps` steps for each candidate tile shape, then persistent grid sizes
+ * for the best one, keeps the fastest (best = {tx, ty, tz, grid_blocks}) and
+ * re-initialises the state to rest.
+ */
+int lbm_batch_autotune(lbm_batch* b, int32_t steps, void* stream, int32_t* best /* 4 */);
+/* out = {cases, real nodes, padded nodes, tiles, tx, ty, tz, grid_blocks, graph_steps}. */
+int lbm_batch_info(lbm_batch* b, int64_t* out /* 9 */);
 
 #ifdef __cplusplus
 }

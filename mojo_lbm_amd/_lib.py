@@ -17,7 +17,8 @@ COLL_SRT, COLL_TRT, COLL_KBC, COLL_RLBM = 0, 1, 2, 3
 STORE_F32, STORE_F16C, STORE_F64 = 0, 1, 2
 LES_NONE, LES_SMAGORINSKY = 0, 1
 U_XYZD, U_DXYZ = 0, 1
-OPT_XCD_REMAP = 1
+OPT_XCD_REMAP, OPT_STATIC_FLAGS = 1, 2
+BATCH_OPT_GRID_BLOCKS, BATCH_OPT_GRAPH_STEPS = 2, 3
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NO_DEVICE = 0, -1, -2, -3, -4
 
 
@@ -68,6 +69,20 @@ SIGNATURES = {
     "lbm_memcpy_d2h": ([_vp, _vp, _sz, _vp], _i),
     "lbm_memset": ([_vp, _i, _sz, _vp], _i),
     "lbm_synchronize": ([_vp], _i),
+    "lbm_batch_create": ([C.POINTER(LbmDesc), C.c_int32, _vp, _vp, C.POINTER(_vp)], _i),
+    "lbm_batch_destroy": ([_vp], _i),
+    "lbm_batch_set_case": ([_vp, C.c_int32, _vp, _vp, _vp, _vp, C.c_uint32], _i),
+    "lbm_batch_set_tile": ([_vp, C.c_int32, C.c_int32, C.c_int32], _i),
+    "lbm_batch_set_option": ([_vp, _i, _i], _i),
+    "lbm_batch_init_equilibrium": ([_vp, _vp], _i),
+    "lbm_batch_run": ([_vp, C.c_int64, _vp], _i),
+    "lbm_batch_force": ([_vp, _vp, _vp], _i),
+    "lbm_batch_case_desc": ([_vp, C.c_int32, C.POINTER(LbmDesc)], _i),
+    "lbm_batch_export_case": ([_vp, C.c_int32, _vp, _vp, _vp, _vp, C.POINTER(C.c_uint32)], _i),
+    "lbm_batch_get_f": ([_vp, C.c_int32, _vp, _vp], _i),
+    "lbm_batch_set_f": ([_vp, C.c_int32, _vp, _vp], _i),
+    "lbm_batch_autotune": ([_vp, C.c_int32, _vp, _vp], _i),
+    "lbm_batch_info": ([_vp, _vp], _i),
 }
 
 _lib = None

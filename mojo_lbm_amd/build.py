@@ -34,8 +34,17 @@ HIP_FLAGS = [
 ]
 
 
-def _compile(src: Path) -> Path:
+def _up_to_date(src: Path, obj: Path) -> bool:
+    if not obj.exists():
+        return False
+    deps = [src, *CSRC.glob("*.hpp"), PKG.parent / "include" / "lbm_hip.h"]
+    return obj.stat().st_mtime > max(d.stat().st_mtime for d in deps if d.exists())
+
+
+def _compile(src: Path, incremental: bool = False) -> Path:
     obj = BUILD / (src.stem + ".o")
+    if incremental and _up_to_date(src, obj):
+        return obj
     cmd = [HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
@@ -43,13 +52,15 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> Path:
-    """Compiles every HIP source and links liblbm_hip.so; returns its path."""
+def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> Path:
+    """Compiles every HIP source (only out-of-date objects when incremental) and links
+    liblbm_hip.so; returns its path."""
     BUILD.mkdir(exist_ok=True)
-    srcs = sorted(CSRC.glob("*.hip"))
+    # the instantiation units of the larger velocity sets first: they take the longest
+    srcs = sorted(CSRC.glob("*.hip"), key=lambda p: (-("d3q27" in p.name) - ("d3q19" in p.name), p.name))
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -62,4 +73,5 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build(int(sys.argv[1]) if len(sys.argv) > 1 else None)
+    args = [a for a in sys.argv[1:] if not a.startswith("-")]
+    build(int(args[0]) if args else None, incremental="--incremental" in sys.argv)

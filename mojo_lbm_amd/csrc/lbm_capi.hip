@@ -21,7 +21,7 @@ int check_launch(const char* what) {
     return LBM_OK;
 }
 
-static int hip_rc(hipError_t e, const char* what) {
+int hip_rc(hipError_t e, const char* what) {
     if (e != hipSuccess) return set_error(LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     return LBM_OK;
 }
@@ -36,7 +36,7 @@ static int log2i(int v) {
 // Validation of a run description: the reference's comptime asserts and
 // host checks (grid.mojo:392-440, config.mojo:100-150, esoteric_pull.mojo:76-78,
 // KBC_.mojo:220) plus this library's own limits.
-static int validate(const lbm_desc* d) {
+int validate(const lbm_desc* d) {
     if (!d) return set_error(LBM_ERR_INVALID, "desc is NULL");
     const bool lat_ok = (d->D == 2 && d->Q == 9) || (d->D == 3 && (d->Q == 19 || d->Q == 27));
     if (!lat_ok) return set_error(LBM_ERR_INVALID, "lattice must be D2Q9, D3Q19 or D3Q27");
@@ -71,7 +71,7 @@ static int validate(const lbm_desc* d) {
     return LBM_OK;
 }
 
-static Geo make_geo(const lbm_desc* d) {
+Geo make_geo(const lbm_desc* d) {
     Geo g{};
     g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
     g.txb = log2i(d->tx); g.tyb = log2i(d->ty); g.tzb = log2i(d->tz);
@@ -359,6 +359,13 @@ int lbm_set_option(lbm_handle* hh, int option, int value) {
         if (value && h.g.n_tiles % 8 != 0)
             return set_error(LBM_ERR_INVALID, "the XCD remap needs a tile count divisible by 8");
         h.g.xcd_chunk = value ? h.g.n_tiles / 8 : 0;
+        return LBM_OK;
+    }
+    if (option == LBM_OPT_STATIC_FLAGS) {
+        if (value != 0 && value != 1) return set_error(LBM_ERR_INVALID, "LBM_OPT_STATIC_FLAGS takes 0 or 1");
+        h.static_flags = value != 0;
+        h.summary_flags = nullptr;  // the next step rebuilds the summary
+        h.summary_bc = nullptr;
         return LBM_OK;
     }
     return set_error(LBM_ERR_INVALID, "unknown option");

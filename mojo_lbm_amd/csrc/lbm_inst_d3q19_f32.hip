@@ -1,6 +1,6 @@
 // Kernel instantiations for one (velocity set, storage) pair; split per file so
 // the library builds in parallel.
-#include "lbm_launch.hpp"
+#include "lbm_launch_impl.hpp"
 
 namespace lbm {
 template struct Launch<Lattice<19>, StoreF32>;

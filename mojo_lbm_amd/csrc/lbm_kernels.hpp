@@ -151,9 +151,9 @@ template <class L> struct TileCtx {
     uint8_t own;
     uint32_t hv[MAXR];    // fetched halo flags
 
-    __device__ __forceinline__ void init_pos(const Geo& g) {
+    __device__ __forceinline__ void init_pos(const Geo& g, uint32_t tile_id) {
         constexpr int D = L::D, Q = L::Q;
-        tile = (uint32_t)__builtin_amdgcn_readfirstlane(tile_of_block(g));
+        tile = (uint32_t)__builtin_amdgcn_readfirstlane(tile_id);
         tid = threadIdx.x;
         const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
         const int lx = tid & (tx - 1), ly = (tid >> g.txb) & (ty - 1), lz = tid >> (g.txb + g.tyb);
@@ -559,13 +559,17 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
 
 // double_buffer_kernel (src/lbm/kernels/double_buffer.mojo:25-96) and
 // esoteric_pull_kernel[is_even] (src/lbm/kernels/esoteric_pull.mojo:24-111).
-template <class L, class S, int MODE, int COLL, bool LES, bool DDF, bool BIG>
-__global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParams<typename S::R> p) {
+//
+// step_tile updates one tile; force_slot numbers its per-wave force records. LINKS
+// compiles the interpolated bounce-back path in (double buffer only): kept out of the
+// plain kernels, whose register budget it would otherwise set.
+template <class L, class S, int MODE, int COLL, bool LES, bool DDF, bool BIG, bool LINKS>
+__device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, uint32_t tile, uint32_t force_slot,
+                                          uint8_t* sfl) {
     using R = typename S::R;
     constexpr int Q = L::Q;
-    extern __shared__ uint8_t sfl[];
     TileCtx<L> t;
-    t.init_pos(p.g);
+    t.init_pos(p.g, tile);
     const uint32_t kind = p.summary ? boundary_tile<L::D>(p.g, p.summary, t.bx, t.by, t.bz) : 3u;
     const bool boundary = (kind & 1u) != 0;
     const int moving = p.moving && (kind & 2u);
@@ -574,7 +578,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParam
     if (boundary) t.fetch_flags(p.g, p.flags);
     R fv[Q];
     load_populations<L, S, MODE, BIG>(t, fin, fv);
-    const bool links = MODE == MODE_DB && p.link_nodes != nullptr;  // uniform
+    constexpr bool links = LINKS && MODE == MODE_DB;
     double force[3] = {0.0, 0.0, 0.0};
     bool active = true;
     if (boundary) {
@@ -605,9 +609,16 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParam
         for (int d = 0; d < L::D; ++d) {
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) force[d] += __shfl_down(force[d], off, 64);
-            if (lane == 0) p.force_partial[((size_t)t.tile * waves + wave) * L::D + d] = force[d];
+            if (lane == 0) p.force_partial[((size_t)force_slot * waves + wave) * L::D + d] = force[d];
         }
     }
+}
+
+template <class L, class S, int MODE, int COLL, bool LES, bool DDF, bool BIG, bool LINKS>
+__global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParams<typename S::R> p) {
+    extern __shared__ uint8_t sfl[];
+    const uint32_t tile = (uint32_t)tile_of_block(p.g);
+    step_tile<L, S, MODE, COLL, LES, DDF, BIG, LINKS>(p, tile, tile, sfl);
 }
 
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125) and the FP64 moment
@@ -619,7 +630,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroPar
     extern __shared__ uint8_t sfl[];
     __shared__ double red[D + 1][16];
     TileCtx<L> t;
-    t.init_pos(p.g);
+    t.init_pos(p.g, (uint32_t)tile_of_block(p.g));
     const Field<S, BIG> f(p.f, p.f_bytes);
     t.fetch_flags(p.g, p.flags);
     R fv[Q];

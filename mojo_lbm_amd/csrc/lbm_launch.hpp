@@ -1,7 +1,5 @@
-// lbm_launch.hpp -- host-side launchers: one Launch<Lattice, Storage> per
-// (velocity set, population storage); each member picks the kernel
-// instantiation for the run's method, collision, LES and DDF toggles and the
-// addressing width.
+// lbm_launch.hpp -- the handle and the declarations of the host-side launchers,
+// one Launch<Lattice, Storage> per (velocity set, population storage).
 #pragma once
 
 #include <string>
@@ -19,6 +17,9 @@ struct Handle {
     bool big = false;            // f_bytes >= 2^31: 64-bit addressing
     double* partial = nullptr;   // per-tile reduction partials (n_tiles * (D+1))
     uint32_t* summary = nullptr; // per-tile region masks, rebuilt before every step
+    bool static_flags = false;   // LBM_OPT_STATIC_FLAGS: rebuild only for new flags / bc buffers
+    mutable const void* summary_flags = nullptr;  // buffers the summary was built from
+    mutable const void* summary_bc = nullptr;
     uint32_t* link_nodes = nullptr;  // interpolated bounce-back records (handle-owned copies)
     float* link_q = nullptr;
     uint32_t n_links = 0;
@@ -28,146 +29,21 @@ struct Handle {
 
 int set_error(int code, const std::string& msg);
 int check_launch(const char* what);
+int hip_rc(hipError_t e, const char* what);
+int validate(const lbm_desc* d);  // description limits; sets the error text
+Geo make_geo(const lbm_desc* d);
 
+// Entry points of one (velocity set, storage) pair. Defined in lbm_launch_impl.hpp and
+// instantiated once per pair in lbm_inst_*.hip; other units see only these declarations.
 template <class L, class S> struct Launch {
-    using R = typename S::R;
-    using T = typename S::T;
-
-    template <int MODE, int COLL, bool LES, bool DDF>
-    static int step_k(const Handle& h, const StepParams<R>& p, hipStream_t st) {
-        if (h.big)
-            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, true>), dim3(h.g.n_tiles),
-                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
-        else
-            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, false>), dim3(h.g.n_tiles),
-                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
-        return check_launch("step_kernel");
-    }
-
-    template <int MODE>
-    static int step_mode(const Handle& h, const StepParams<R>& p, hipStream_t st) {
-        const bool kbc = h.d.collision == LBM_COLL_KBC;
-        const bool les = h.d.les == LBM_LES_SMAGORINSKY;
-        const bool ddf = h.d.ddf_shift != 0;
-        if (!kbc && !les && !ddf) return step_k<MODE, C_SRT, false, false>(h, p, st);
-        if (!kbc && !les && ddf) return step_k<MODE, C_SRT, false, true>(h, p, st);
-        if (!kbc && les && !ddf) return step_k<MODE, C_SRT, true, false>(h, p, st);
-        if (!kbc && les && ddf) return step_k<MODE, C_SRT, true, true>(h, p, st);
-        if (kbc && !les && !ddf) return step_k<MODE, C_KBC, false, false>(h, p, st);
-        if (kbc && !les && ddf) return step_k<MODE, C_KBC, false, true>(h, p, st);
-        if (kbc && les && !ddf) return step_k<MODE, C_KBC, true, false>(h, p, st);
-        return step_k<MODE, C_KBC, true, true>(h, p, st);
-    }
-
     static int step(const Handle& h, void* f_out, const void* f_in, const void* bc, const uint8_t* flags, double tau,
-                    int mode, hipStream_t st) {
-        if ((uintptr_t)flags & 15u) return set_error(LBM_ERR_INVALID, "flags must be 16-byte aligned");
-        const uint32_t n_chunks = (uint32_t)(((size_t)h.g.n_tiles * h.g.tile_cells) >> 4);
-        hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, flags,
-                           (const R*)bc, (int)(h.d.include_moving_boundary != 0), h.summary, h.g, n_chunks);
-        if (int rc = check_launch("tile_summary_kernel")) return rc;
-        StepParams<R> p;
-        p.f_in = f_in;
-        p.f_out = f_out;
-        p.bc = (const R*)bc;
-        p.flags = flags;
-        p.summary = h.summary;
-        p.tau = (R)tau;
-        p.inv_tau = (R)1 / p.tau;
-        p.cs = (R)h.d.smagorinsky_cs;
-        p.moving = h.d.include_moving_boundary != 0;
-        p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
-        p.f_bytes = (uint32_t)(h.big ? 0 : h.f_bytes);
-        p.link_nodes = h.n_links ? h.link_nodes : nullptr;
-        p.link_q = h.link_q;
-        p.n_links = h.n_links;
-        p.force_partial = h.force_partial;
-        p.g = h.g;
-        if (mode == MODE_DB) return step_mode<MODE_DB>(h, p, st);
-        if (mode == MODE_EP_EVEN) return step_mode<MODE_EP_EVEN>(h, p, st);
-        return step_mode<MODE_EP_ODD>(h, p, st);
-    }
-
-    template <int MODE, bool DDF, bool REDUCE>
-    static int macro_k(const Handle& h, const MacroParams<R>& p, hipStream_t st) {
-        if (h.big)
-            hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, REDUCE, true>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
-                               h.lds_bytes, st, p);
-        else
-            hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, REDUCE, false>), dim3(h.g.n_tiles),
-                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
-        return check_launch("macro_kernel");
-    }
-
-    template <bool REDUCE>
-    static int macro_any(const Handle& h, const MacroParams<R>& p, int mode, hipStream_t st) {
-        const bool ddf = h.d.ddf_shift != 0;
-        if (mode == MODE_DB) return ddf ? macro_k<MODE_DB, true, REDUCE>(h, p, st) : macro_k<MODE_DB, false, REDUCE>(h, p, st);
-        if (mode == MODE_EP_EVEN)
-            return ddf ? macro_k<MODE_EP_EVEN, true, REDUCE>(h, p, st) : macro_k<MODE_EP_EVEN, false, REDUCE>(h, p, st);
-        return ddf ? macro_k<MODE_EP_ODD, true, REDUCE>(h, p, st) : macro_k<MODE_EP_ODD, false, REDUCE>(h, p, st);
-    }
-
-    static MacroParams<R> macro_params(const Handle& h, const void* f, const void* bc, const uint8_t* flags,
-                                       bool mode_is_db) {
-        MacroParams<R> p{};
-        p.f = f;
-        p.bc = (const R*)bc;
-        p.flags = flags;
-        p.moving = h.d.include_moving_boundary != 0;
-        p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
-        p.f_bytes = (uint32_t)(h.big ? 0 : h.f_bytes);
-        p.g = h.g;
-        if (mode_is_db && h.n_links) {
-            p.link_nodes = h.link_nodes;
-            p.link_q = h.link_q;
-            p.n_links = h.n_links;
-        }
-        return p;
-    }
-
+                    int mode, hipStream_t st);
     static int macro(const Handle& h, void* rho, void* u, const void* f, const void* bc, const uint8_t* flags,
-                     int mode, int u_layout, hipStream_t st) {
-        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
-        p.rho = (R*)rho;
-        p.u = (R*)u;
-        p.u_layout = u_layout;
-        return macro_any<false>(h, p, mode, st);
-    }
-
+                     int mode, int u_layout, hipStream_t st);
     static int reduce(const Handle& h, double* out, const void* f, const void* bc, const uint8_t* flags, int mode,
-                      hipStream_t st) {
-        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
-        p.partial = h.partial;
-        int rc = macro_any<true>(h, p, mode, st);
-        if (rc) return rc;
-        if (L::D == 2) hipLaunchKernelGGL((reduce_finish<3>), dim3(1), dim3(256), 0, st, h.partial, h.g.n_tiles, out);
-        else hipLaunchKernelGGL((reduce_finish<4>), dim3(1), dim3(256), 0, st, h.partial, h.g.n_tiles, out);
-        return check_launch("reduce_finish");
-    }
-
-    static int link_force(const Handle& h, double* out, hipStream_t st) {
-        const int waves = (h.g.tile_cells + 63) / 64;
-        const int n = h.g.n_tiles * waves;
-        if (L::D == 2) hipLaunchKernelGGL((reduce_finish<2>), dim3(1), dim3(256), 0, st, h.force_partial, n, out);
-        else hipLaunchKernelGGL((reduce_finish<3>), dim3(1), dim3(256), 0, st, h.force_partial, n, out);
-        return check_launch("reduce_finish");
-    }
-
-    static int init(const Handle& h, void* f, double rho_d, const double* u_d, hipStream_t st) {
-        QVals<R> v{};
-        R u[3] = {0, 0, 0};
-        if (u_d)
-            for (int d = 0; d < L::D; ++d) u[d] = (R)u_d[d];
-        const R rho = (R)rho_d;
-        const R uu = dot_uu<L, R>(u);
-        const bool ddf = h.d.ddf_shift != 0;
-        for (int q = 0; q < L::Q; ++q) v.v[q] = ddf ? feq<L, R, true>(q, rho, u, uu) : feq<L, R, false>(q, rho, u, uu);
-        const size_t n = (size_t)h.g.n_tiles * h.g.tile_cells * L::Q;
-        const unsigned blocks = (unsigned)((n + 255) / 256);
-        hipLaunchKernelGGL((fill_kernel<S>), dim3(blocks), dim3(256), 0, st, (T*)f, v, L::Q, (uint32_t)h.g.tile_cells, n);
-        return check_launch("fill_kernel");
-    }
+                      hipStream_t st);
+    static int link_force(const Handle& h, double* out, hipStream_t st);
+    static int init(const Handle& h, void* f, double rho_d, const double* u_d, hipStream_t st);
 };
 
 // Launchers are instantiated once per (lattice, storage) in lbm_inst_*.hip.

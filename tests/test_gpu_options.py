@@ -1,0 +1,44 @@
+"""Launch options of a handle: the XCD tile order and static flags change nothing in the
+results (bit for bit), and static flags still pick up a new flags buffer."""
+import numpy as np
+import pytest
+
+from mojo_lbm_amd import _lib
+from oracle import oracle as O
+from tests import _util as U
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("option,value", [(_lib.OPT_STATIC_FLAGS, 1), (_lib.OPT_XCD_REMAP, 0)])
+def test_option_keeps_results(gpu, option, value):
+    desc = U.make_desc("D3Q19", (32, 32, 32))
+    f, bc, flags = U.perturbed_state(desc, seed=4)
+    a = U.GpuRun(desc, f, bc, flags)
+    b = U.GpuRun(desc, f, bc, flags)
+    _lib.check(b.L.lbm_set_option(b.h.ptr, option, value))
+    for _ in range(5):
+        a.step_db(0.6)
+        b.step_db(0.6)
+    assert np.array_equal(a.f_host(), b.f_host())
+
+
+def test_static_flags_follow_a_new_flags_buffer(gpu):
+    import torch
+
+    desc = U.make_desc("D3Q19", (16, 16, 16))
+    f, bc, flags = U.perturbed_state(desc, seed=8)
+    run = U.GpuRun(desc, f, bc, flags)
+    _lib.check(run.L.lbm_set_option(run.h.ptr, _lib.OPT_STATIC_FLAGS, 1))
+    run.step_db(0.7)
+    # an obstacle appears in a second flags buffer: the summary must be rebuilt for it
+    flags2 = flags.copy()
+    amap = U.index_map(desc, 1)[..., 0]
+    flags2[amap[5:9, 5:9, 5:9].ravel()] = 1
+    run.flags = torch.from_numpy(flags2).cuda()
+    run.step_db(0.7)
+    ref = f.copy()
+    tmp = f.copy()
+    O.step_db(desc, tmp, ref, bc, flags, 0.7)
+    O.step_db(desc, ref, tmp, bc, flags2, 0.7)
+    assert np.array_equal(run.f_host(), ref)

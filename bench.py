@@ -338,7 +338,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default=None, help="run one variant: fp32 | f16c | f16c_ep | fp32_ep")
+    ap.add_argument("--only", default=None,
+                    help="run one variant: fp32 | f16c | f16c_ep | fp32_ep | fp32_periodic | f16c_periodic | cylinders")
     ap.add_argument("--xcd-remap", type=int, default=None, help="override LBM_OPT_XCD_REMAP (0 or 1)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain 1 GiB -> 1 GiB device copy kernel (for scaling rocprofv3 byte counters)")
@@ -349,8 +350,8 @@ def main():
     ap.add_argument("--cylinder-compare", action="store_true",
                     help="run only the batched cylinder cases, then every case alone, and compare")
     args = ap.parse_args()
-    if args.cylinder_compare:
-        print(json.dumps(cylinder_batch(args.cylinders or 20000, compare=True)), flush=True)
+    if args.cylinder_compare or args.only == "cylinders":
+        print(json.dumps(cylinder_batch(args.cylinders or 20000, compare=args.cylinder_compare)), flush=True)
         return
     if args.accuracy:
         accuracy_study(args.n, [int(s) for s in args.accuracy.split(",")])

@@ -114,3 +114,31 @@ def test_cfg3_f16c_ddf_cube(gpu, method):
     tot = lbm.reduce_moments(solver, asm, after_odd_step=True)
     # FP64 totals: the GPU sums in a tree, the oracle sequentially; near-zero components need an absolute bound
     np.testing.assert_allclose(tot, O.reduce_moments(desc, ref, bc, flags, after_odd=True), rtol=1e-12, atol=1e-10)
+
+
+def test_dfg_re20_cylinder_drag(gpu):
+    """Schaefer-Turek (DFG) 2-D cylinder at Re = 20, the reference's cylinder benchmark set-up
+    (benchmarks/Cylinder2D/CylinderFlow_Re20.mojo:96-150: N = ny nodes over H = 0.41, nx = 5 N,
+    centre node floor(0.2 / dx), parabolic inlet, EQUILIBRIUM outlet), here at N = 128 with
+    interpolated bounce-back on the cylinder. Published Cd = 5.57953523384, Cl = 0.010618948146
+    (:59-60); the reference reports 1.2 % / 0.17 % / 0.35 % drag error at N = 256 / 512 / 1024
+    (:68-91). The drag must be within 1 % and steady; the lift positive and of the published size."""
+    from mojo_lbm_amd.batch import CaseBatch
+    from mojo_lbm_amd.cylinder import CylinderCase
+
+    lat = lbm.get_D2Q9()
+    N = 128
+    dx = 0.41 / (N - 1)
+    cen = float(np.floor(0.2 / dx))
+    case = CylinderCase(nx=5 * N, ny=N, diameter=0.1 / dx, cx=cen, cy=cen, re=20.0, u_max=0.06)
+    b = CaseBatch(lat, [(case.nx, case.ny, 1)], [case.tau], lbm.DoubleBufferConfig(BCs={lbm.Flags.EQUILIBRIUM}))
+    flags, bc, links = case.arrays(lat)
+    b.set_case(0, flags, bc, links)
+    b.init_equilibrium()
+    b.run(50000)
+    cd0, _ = case.coefficients(b.force()[0])
+    b.run(10000)
+    cd, cl = case.coefficients(b.force()[0])
+    assert abs(cd - cd0) < 1e-4 * cd
+    assert abs(cd - 5.57953523384) < 0.01 * 5.57953523384, cd
+    assert 0.0 < cl < 0.05, cl

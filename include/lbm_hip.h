@@ -42,8 +42,9 @@ enum { LBM_COLL_SRT = 0, LBM_COLL_TRT = 1, LBM_COLL_KBC = 2, LBM_COLL_RLBM = 3 }
  * or f64 (the FP64 reference runs). */
 enum { LBM_STORE_F32 = 0, LBM_STORE_F16C = 1, LBM_STORE_F64 = 2 };
 
-/* Turbulence closure applied in collide (src/lbm/kernels/steps/collide.mojo:66-73). */
-enum { LBM_LES_NONE = 0, LBM_LES_SMAGORINSKY = 1 };
+/* Turbulence closure applied in collide (src/lbm/kernels/steps/collide.mojo:66-73).
+ * LBM_LES_MLP is the learned closure of lbm_set_mlp (3-D lattices; no reference counterpart). */
+enum { LBM_LES_NONE = 0, LBM_LES_SMAGORINSKY = 1, LBM_LES_MLP = 2 };
 
 /* Node flags: enum in the low 4 bits, bit masks above (src/lbm/constants.mojo:58-117). */
 enum {
@@ -198,6 +199,35 @@ int lbm_link_force(lbm_handle* h, double* out_dev, void* stream);
  * converting fields in and out of the population format.
  */
 int lbm_f16c_convert(const void* in, void* out, size_t n, int to_f16c, void* stream);
+
+/*
+ * Parameters of the learned LES closure used by desc.les == LBM_LES_MLP: replaces the
+ * Smagorinsky eddy viscosity of get_Smagorinsky_LES_tau at
+ * src/lbm/kernels/steps/collide.mojo:70-73 with an MLP 9 -> 32 -> 32 -> 1 (SiLU, SiLU,
+ * softplus) of (Pi_neq xx, xy, xz, yy, yz, zz; rho; |u|; Smagorinsky nu_t), and falls back
+ * to Smagorinsky where a feature lies outside the training range. params: n = 1446 host
+ * floats: W1 (9x32, k-major), b1[32], W2 (32x32, k-major), b2[32], w3[32], b3, mean[9],
+ * inv_std[9], lo[9], hi[9], nu_scale. Copied into the handle. No reference counterpart.
+ */
+int lbm_set_mlp(lbm_handle* h, const float* params, int32_t n);
+
+/*
+ * Training kernels of the closure (device arrays; no reference counterpart). X: n x 9 raw
+ * features, row-major; y: n target eddy viscosities; params: the 1446-float block above
+ * (only the first 1409, the weights, are trained; normalisation and nu_scale are read).
+ *   lbm_mlp_forward:   nu[s] = softplus(MLP(x_s)) * nu_scale (no range fallback).
+ *   lbm_mlp_loss_grad: loss = mean_s (softplus(MLP(x_s)) - y_s / nu_scale)^2 into *loss_dev
+ *                      (double, may be NULL) and d loss / d weights into grad_dev (1409
+ *                      floats); deterministic; needs lbm_mlp_workspace_bytes(n) of scratch.
+ *   lbm_mlp_adam:      one Adam step (bias-corrected, step t >= 1) on the 1409 weights with
+ *                      moment buffers m, v (1409 floats each, zero before t = 1).
+ */
+int lbm_mlp_forward(const float* params_dev, const float* X_dev, int64_t n, float* nu_dev, void* stream);
+size_t lbm_mlp_workspace_bytes(int64_t n);
+int lbm_mlp_loss_grad(const float* params_dev, const float* X_dev, const float* y_dev, int64_t n,
+                      float* grad_dev, double* loss_dev, void* workspace_dev, void* stream);
+int lbm_mlp_adam(float* params_dev, float* m_dev, float* v_dev, const float* grad_dev, int32_t t,
+                 float lr, float beta1, float beta2, float eps, void* stream);
 
 /*
  * Launch tuning of one handle (no reference counterpart: the reference fixes

@@ -15,7 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "liblbm_hip.so"
 DOUBLE_BUFFER, ESOTERIC_PULL = 0, 1
 COLL_SRT, COLL_TRT, COLL_KBC, COLL_RLBM = 0, 1, 2, 3
 STORE_F32, STORE_F16C, STORE_F64 = 0, 1, 2
-LES_NONE, LES_SMAGORINSKY = 0, 1
+LES_NONE, LES_SMAGORINSKY, LES_MLP = 0, 1, 2
 U_XYZD, U_DXYZ = 0, 1
 OPT_XCD_REMAP, OPT_STATIC_FLAGS = 1, 2
 BATCH_OPT_GRID_BLOCKS, BATCH_OPT_GRAPH_STEPS = 2, 3
@@ -60,6 +60,11 @@ SIGNATURES = {
     "lbm_init_equilibrium": ([_vp, _vp, _d, C.POINTER(_d), _vp], _i),
     "lbm_set_exterior_wall": ([_vp, _vp, _vp, _cp, _i, C.POINTER(_d), _d, _vp], _i),
     "lbm_set_links": ([_vp, _vp, _vp, C.c_uint32], _i),
+    "lbm_set_mlp": ([_vp, _vp, C.c_int32], _i),
+    "lbm_mlp_forward": ([_vp, _vp, C.c_int64, _vp, _vp], _i),
+    "lbm_mlp_workspace_bytes": ([C.c_int64], _sz),
+    "lbm_mlp_loss_grad": ([_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp], _i),
+    "lbm_mlp_adam": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, _vp], _i),
     "lbm_link_force": ([_vp, _vp, _vp], _i),
     "lbm_f16c_convert": ([_vp, _vp, _sz, _i, _vp], _i),
     "lbm_set_option": ([_vp, _i, _i], _i),

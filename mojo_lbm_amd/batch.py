@@ -39,7 +39,7 @@ class CaseBatch:
         self.proto = _lib.LbmDesc(
             D=self.D, Q=self.Q, nx=0, ny=0, nz=0, tx=tile[0], ty=tile[1], tz=tile[2], method=config.lbm_method,
             collision=Collisions.ids[config.collision_op], storage=config.storage_id, ddf_shift=int(config.DDF_shift),
-            les=int(config.LES), include_moving_boundary=int(config.include_moving_boundary),
+            les=config.les_id, include_moving_boundary=int(config.include_moving_boundary),
             included_bcs=config.included_bcs_mask, smagorinsky_cs=config.Cs)
         self.L = _lib.lib()
         ptr = C.c_void_p()

@@ -19,12 +19,15 @@ class LBM_Config:
     def __init__(self, lbm_method=LBM_method.DOUBLE_BUFFER, *, collision_op: str = "SRT", LES: bool = False,
                  BCs=frozenset(), DDF_shift: bool = False, use_float16c: bool = False, f_dtype=None,
                  include_moving_boundary: bool = True, capture_density: bool = False,
-                 capture_velocity: bool = False, Cs: float = 0.1):
+                 capture_velocity: bool = False, Cs: float = 0.1, LES_model: str = "smagorinsky"):
         if lbm_method not in LBM_method.valid_set:
             raise ValueError(f"lbm_method not valid: {lbm_method}")
         self.lbm_method = lbm_method
         self.collision_op = collision_op
         self.LES = bool(LES)
+        if LES_model not in ("smagorinsky", "mlp"):
+            raise ValueError(f"LES_model must be 'smagorinsky' or 'mlp', not {LES_model!r}")
+        self.LES_model = LES_model
         self.DDF_shift = bool(DDF_shift)
         self.use_float16c = bool(use_float16c)
         self.include_moving_boundary = bool(include_moving_boundary)
@@ -43,6 +46,13 @@ class LBM_Config:
             raise ValueError(f"Invalid collision operator {collision_op!r}")
         if capture_density or capture_velocity:
             raise NotImplementedError("capture_density / capture_velocity are not provided by this build")
+
+    @property
+    def les_id(self) -> int:
+        """lbm_desc.les: 0 none, 1 Smagorinsky (the reference's LES), 2 the learned closure."""
+        if not self.LES:
+            return 0
+        return 2 if self.LES_model == "mlp" else 1
 
     def implies_f_noneq(self, *, force: bool = False) -> bool:
         return self.LES or self.collision_op in Collisions.that_need_fneq or force

@@ -338,6 +338,7 @@ int lbm_batch_create(const lbm_desc* proto, int32_t n_cases, const int32_t* size
     if (!proto || !sizes || !taus || n_cases <= 0) return set_error(LBM_ERR_INVALID, "empty or NULL case list");
     if (proto->method != LBM_DOUBLE_BUFFER)
         return set_error(LBM_ERR_UNSUPPORTED, "batched cases use the double-buffer method");
+    if (proto->les == LBM_LES_MLP) return set_error(LBM_ERR_UNSUPPORTED, "batched cases take no learned closure");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(LBM_ERR_NO_DEVICE, "no HIP device visible");
     Batch* b = new Batch();

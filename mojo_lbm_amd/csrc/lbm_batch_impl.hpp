@@ -13,7 +13,7 @@ template <class R> struct BatchCaseRec {
     uint32_t tile_begin;  // first batch tile of the case
 };
 
-template <class L, class S, int COLL, bool LES, bool DDF, bool LINKS>
+template <class L, class S, int COLL, int LES, bool DDF, bool LINKS>
 __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
     batch_step_kernel(const BatchCaseRec<typename S::R>* __restrict__ cases, const uint32_t* __restrict__ tile_case,
                       uint32_t n_tiles, uint32_t chunk, int parity) {
@@ -41,7 +41,7 @@ template <class L, class S> struct BatchLaunchImpl {
     using R = typename S::R;
     using T = typename S::T;
 
-    template <int COLL, bool LES, bool DDF>
+    template <int COLL, int LES, bool DDF>
     static int step_k(const Batch& b, int parity, hipStream_t st) {
         const uint32_t slots = 8u * b.chunk;
         const uint32_t grid = b.grid_blocks > 0 ? std::min<uint32_t>((uint32_t)b.grid_blocks, slots) : slots;

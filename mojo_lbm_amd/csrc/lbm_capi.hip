@@ -64,7 +64,10 @@ int validate(const lbm_desc* d) {
     if (d->collision != LBM_COLL_SRT && d->collision != LBM_COLL_KBC)
         return set_error(LBM_ERR_INVALID, "unknown collision operator");
     if (d->storage < LBM_STORE_F32 || d->storage > LBM_STORE_F64) return set_error(LBM_ERR_INVALID, "unknown storage");
-    if (d->les != LBM_LES_NONE && d->les != LBM_LES_SMAGORINSKY) return set_error(LBM_ERR_INVALID, "unknown LES model");
+    if (d->les != LBM_LES_NONE && d->les != LBM_LES_SMAGORINSKY && d->les != LBM_LES_MLP)
+        return set_error(LBM_ERR_INVALID, "unknown LES model");
+    if (d->les == LBM_LES_MLP && d->D != 3)
+        return set_error(LBM_ERR_UNSUPPORTED, "the learned closure needs a 3-D lattice");
     if (!(d->smagorinsky_cs >= 0.0)) return set_error(LBM_ERR_INVALID, "smagorinsky_cs must be >= 0");
     const double elems = (double)d->nx * d->ny * d->nz * d->Q;
     if (elems >= 4294967296.0) return set_error(LBM_ERR_UNSUPPORTED, "more than 2^32 populations per buffer");
@@ -186,6 +189,7 @@ int lbm_destroy(lbm_handle* hh) {
     if (h->link_nodes) (void)hipFree(h->link_nodes);
     if (h->link_q) (void)hipFree(h->link_q);
     if (h->force_partial) (void)hipFree(h->force_partial);
+    if (h->mlp) (void)hipFree(h->mlp);
     delete h;
     return LBM_OK;
 }
@@ -333,6 +337,18 @@ int lbm_set_links(lbm_handle* hh, const uint32_t* nodes, const float* qdist, uin
         return set_error(LBM_ERR_HIP, "copy of the link records failed");
     h.n_links = n_links;
     return LBM_OK;
+}
+
+int lbm_set_mlp(lbm_handle* hh, const float* params, int32_t n) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (!params || n != MLP_TOTAL) return set_error(LBM_ERR_INVALID, "the closure takes 1446 parameters");
+    for (int32_t i = 0; i < n; ++i)
+        if (!std::isfinite(params[i])) return set_error(LBM_ERR_INVALID, "non-finite closure parameter");
+    if (!h.mlp && hipMalloc((void**)&h.mlp, MLP_TOTAL * sizeof(float)) != hipSuccess)
+        return set_error(LBM_ERR_HIP, "hipMalloc of the closure parameters failed");
+    if (hipDeviceSynchronize() != hipSuccess) return set_error(LBM_ERR_HIP, "synchronize before replacing the closure");
+    return hip_rc(hipMemcpy(h.mlp, params, MLP_TOTAL * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
 }
 
 int lbm_link_force(lbm_handle* hh, double* out_dev, void* stream) {

@@ -38,6 +38,7 @@ template <class R> struct StepParams {
     const float* link_q;       // (Q-1) link distances per record
     uint32_t n_links;
     double* force_partial;     // per wave: D doubles of momentum-exchange force (when link_nodes != NULL)
+    const float* mlp;          // learned-closure parameter block (LES_MLP)
     Geo g;
 };
 
@@ -391,8 +392,11 @@ __device__ __forceinline__ void halfway_bounce_back(const TileCtx<L>& t, const u
 // a = f*(x, opp q) (already in fv[q] by halfway bounce-back) and
 //   qd <= 1/2: b = f*(x + c_q, opp q), f_q = (2 qd) a + (1 - 2 qd) b  (a when x + c_q is SOLID),
 //   qd >  1/2: b = f*(x, q),           f_q = (0.5 / qd) a + ((2 qd - 1) / (2 qd)) b;
-// the link's momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF) is added to force.
-// Returns the rebuilt directions (they take no moving-wall term).
+// the link's momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF) is added to force,
+// and the mass the interpolation adds, dm = sum (f_q - a), is taken from the rest
+// population so the node holds the mass halfway bounce-back gives it (the mass fix of
+// nodewise_interpolated_BB.mojo:170-176, which adds dm instead; subtracting it is what
+// conserves mass). Returns the rebuilt directions (they take no moving-wall term).
 template <class L, class S, bool DDF, bool BIG>
 __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uint8_t* sfl, const Field<S, BIG>& f,
                                                   const uint32_t* __restrict__ link_nodes,
@@ -408,6 +412,7 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uin
     }
     if (lo >= n_links || link_nodes[lo] != node) return 0;
     uint32_t done = 0;
+    R dm = 0;  // mass the interpolation adds, taken back from the rest population
 #pragma unroll
     for (int q = 1; q < Q; ++q) {
         const R qd = (R)link_q[(size_t)lo * (Q - 1) + (q - 1)];
@@ -426,6 +431,7 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uin
             fb = (R)0.5 / qd * a + ((R)2 * qd - (R)1) / ((R)2 * qd) * b;
         }
         fv[q] = fb;
+        dm = dm + (fb - a);
         done |= 1u << q;
         R s = a + fb;
         if (DDF) s = s + (R)2 * weight<L, R>(q);
@@ -435,6 +441,7 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uin
             if (c != 0) force[d] += (double)(c > 0 ? -s : s);
         }
     }
+    if (done) fv[0] = fv[0] - dm;
     return done;
 }
 
@@ -563,7 +570,7 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
 // step_tile updates one tile; force_slot numbers its per-wave force records. LINKS
 // compiles the interpolated bounce-back path in (double buffer only): kept out of the
 // plain kernels, whose register budget it would otherwise set.
-template <class L, class S, int MODE, int COLL, bool LES, bool DDF, bool BIG, bool LINKS>
+template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS>
 __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, uint32_t tile, uint32_t force_slot,
                                           uint8_t* sfl) {
     using R = typename S::R;
@@ -598,7 +605,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
         }
     }
     if (active) {
-        collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs);
+        collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs, p.mlp);
         store_populations<L, S, MODE, BIG>(t, fout, fv);
     }
     if (links) {
@@ -614,7 +621,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
     }
 }
 
-template <class L, class S, int MODE, int COLL, bool LES, bool DDF, bool BIG, bool LINKS>
+template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS>
 __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParams<typename S::R> p) {
     extern __shared__ uint8_t sfl[];
     const uint32_t tile = (uint32_t)tile_of_block(p.g);

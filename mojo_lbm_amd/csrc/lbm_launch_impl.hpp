@@ -11,7 +11,7 @@ template <class L, class S> struct LaunchImpl {
     using R = typename S::R;
     using T = typename S::T;
 
-    template <int MODE, int COLL, bool LES, bool DDF, bool LINKS>
+    template <int MODE, int COLL, int LES, bool DDF, bool LINKS>
     static int step_l(const Handle& h, const StepParams<R>& p, hipStream_t st) {
         if (h.big)
             hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, true, LINKS>), dim3(h.g.n_tiles),
@@ -22,10 +22,26 @@ template <class L, class S> struct LaunchImpl {
         return check_launch("step_kernel");
     }
 
-    template <int MODE, int COLL, bool LES, bool DDF>
+    template <int MODE, int COLL, int LES, bool DDF>
     static int step_k(const Handle& h, const StepParams<R>& p, hipStream_t st) {
         if (MODE == MODE_DB && p.link_nodes) return step_l<MODE, COLL, LES, DDF, true>(h, p, st);
         return step_l<MODE, COLL, LES, DDF, false>(h, p, st);
+    }
+
+    // The learned closure: 3-D lattices, 32-bit addressing, no interpolated links.
+    template <int MODE, int COLL, bool DDF>
+    static int step_mlp(const Handle& h, const StepParams<R>& p, hipStream_t st) {
+        if constexpr (L::D == 3) {
+            if (h.big || p.link_nodes)
+                return set_error(LBM_ERR_UNSUPPORTED,
+                                 "the learned closure runs without interpolated links and with buffers under 2 GiB");
+            if (!p.mlp) return set_error(LBM_ERR_INVALID, "the learned closure needs lbm_set_mlp first");
+            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES_MLP, DDF, false, false>), dim3(h.g.n_tiles),
+                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
+            return check_launch("step_kernel");
+        } else {
+            return set_error(LBM_ERR_UNSUPPORTED, "the learned closure needs a 3-D lattice");
+        }
     }
 
     template <int MODE>
@@ -33,6 +49,12 @@ template <class L, class S> struct LaunchImpl {
         const bool kbc = h.d.collision == LBM_COLL_KBC;
         const bool les = h.d.les == LBM_LES_SMAGORINSKY;
         const bool ddf = h.d.ddf_shift != 0;
+        if (h.d.les == LBM_LES_MLP) {
+            if (!kbc && !ddf) return step_mlp<MODE, C_SRT, false>(h, p, st);
+            if (!kbc && ddf) return step_mlp<MODE, C_SRT, true>(h, p, st);
+            if (kbc && !ddf) return step_mlp<MODE, C_KBC, false>(h, p, st);
+            return step_mlp<MODE, C_KBC, true>(h, p, st);
+        }
         if (!kbc && !les && !ddf) return step_k<MODE, C_SRT, false, false>(h, p, st);
         if (!kbc && !les && ddf) return step_k<MODE, C_SRT, false, true>(h, p, st);
         if (!kbc && les && !ddf) return step_k<MODE, C_SRT, true, false>(h, p, st);
@@ -70,6 +92,7 @@ template <class L, class S> struct LaunchImpl {
         p.link_q = h.link_q;
         p.n_links = h.n_links;
         p.force_partial = h.force_partial;
+        p.mlp = h.mlp;
         p.g = h.g;
         if (mode == MODE_DB) return step_mode<MODE_DB>(h, p, st);
         if (mode == MODE_EP_EVEN) return step_mode<MODE_EP_EVEN>(h, p, st);

@@ -157,13 +157,65 @@ __device__ __forceinline__ void kbc_shear(const R (&pi)[6], R (&s)[L::Q]) {
 }
 
 enum Coll { C_SRT = 0, C_KBC = 2 };
+enum LesMode { LES_OFF = 0, LES_SMAG = 1, LES_MLP = 2 };
+
+// ------------------------------------------------------------ learned closure
+// MLP 9 -> 32 -> 32 -> 1 (SiLU, SiLU, softplus) predicting the eddy viscosity nu_t
+// from x = (Pi_neq in the stress-pair order xx, xy, xz, yy, yz, zz; rho; |u|;
+// Smagorinsky nu_t). Parameter block (floats): W1[k*32+j] (9x32), b1[32],
+// W2[k*32+j] (32x32), b2[32], w3[32], b3, then mean[9], inv_std[9] (input
+// normalisation), lo[9], hi[9] (training range of the raw features) and
+// nu_scale (nu_t = softplus(out) * nu_scale). Outside the training range the
+// Smagorinsky value is used. Replaces get_Smagorinsky_LES_tau at
+// src/lbm/kernels/steps/collide.mojo:70-73; no reference counterpart.
+constexpr int MLP_IN = 9, MLP_H = 32;
+constexpr int MLP_W1 = 0, MLP_B1 = MLP_W1 + MLP_IN * MLP_H, MLP_W2 = MLP_B1 + MLP_H,
+              MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H, MLP_B3 = MLP_W3 + MLP_H,
+              MLP_NPARAM = MLP_B3 + 1;  // 1409 trainable
+constexpr int MLP_MEAN = MLP_NPARAM, MLP_ISTD = MLP_MEAN + MLP_IN, MLP_LO = MLP_ISTD + MLP_IN,
+              MLP_HI = MLP_LO + MLP_IN, MLP_NUSCALE = MLP_HI + MLP_IN, MLP_TOTAL = MLP_NUSCALE + 1;  // 1446
+
+__host__ __device__ __forceinline__ float silu(float x) { return x / (1.0f + expf(-x)); }
+__host__ __device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
+
+// nu_t of the closure for raw features x; returns false when x is outside the training range.
+__device__ __forceinline__ bool mlp_nu(const float* __restrict__ P, const float (&x)[MLP_IN], float& nu) {
+    float z[MLP_IN];
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) {
+        inside = inside && x[k] >= P[MLP_LO + k] && x[k] <= P[MLP_HI + k];
+        z[k] = (x[k] - P[MLP_MEAN + k]) * P[MLP_ISTD + k];
+    }
+    if (!inside) return false;
+    float h1[MLP_H];
+#pragma unroll
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = P[MLP_B1 + j];
+#pragma unroll
+        for (int k = 0; k < MLP_IN; ++k) a = a + P[MLP_W1 + k * MLP_H + j] * z[k];
+        h1[j] = silu(a);
+    }
+    float out = P[MLP_B3];
+#pragma unroll 4
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = P[MLP_B2 + j];
+#pragma unroll
+        for (int k = 0; k < MLP_H; ++k) a = a + P[MLP_W2 + k * MLP_H + j] * h1[k];
+        out = out + P[MLP_W3 + j] * silu(a);
+    }
+    nu = softplus(out) * P[MLP_NUSCALE];
+    return true;
+}
 
 // collide, src/lbm/kernels/steps/collide.mojo:23-90 (SRT: ops/collisions/collision.mojo:12-46,
 // KBC: ops/collisions/KBC_.mojo:180-238).
-template <class L, class R, int COLL, bool LES, bool DDF>
+template <class L, class R, int COLL, int LES, bool DDF>
 // inv_tau_base is 1/tau rounded in R (computed once on the host; equal to the
 // per-node (R)1 / tau the reference evaluates when no LES term changes tau).
-__device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R cs) {
+// LES: LES_OFF, LES_SMAG (the reference's Smagorinsky term) or LES_MLP (the
+// learned closure with parameter block mlp, Smagorinsky outside its range).
+__device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R cs, const float* mlp = nullptr) {
     constexpr int Q = L::Q;
     const R rho = density<L, R, DDF>(f);
     R u[3];
@@ -176,7 +228,18 @@ __device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R c
 #pragma unroll
         for (int q = 0; q < Q; ++q) fn[q] = f[q] - feq<L, R, DDF>(q, rho, u, uu);
         pi_neq<L, R>(fn, pi);
-        if (LES) tau_local = tau_local + smagorinsky_tau<L, R>(pi, rho, tau, cs);
+        if (LES == LES_SMAG) tau_local = tau_local + smagorinsky_tau<L, R>(pi, rho, tau, cs);
+        if (LES == LES_MLP) {
+            const R tau_s = smagorinsky_tau<L, R>(pi, rho, tau, cs);
+            float x[MLP_IN];
+#pragma unroll
+            for (int n = 0; n < 6; ++n) x[n] = n < L::D * (L::D + 1) / 2 ? (float)pi[n] : 0.0f;
+            x[6] = (float)rho;
+            x[7] = (float)sqrt(uu);
+            x[8] = (float)(tau_s / (R)3);
+            float nu;
+            tau_local = mlp_nu(mlp, x, nu) ? tau_local + (R)3 * (R)nu : tau_local + tau_s;
+        }
         if (COLL == C_KBC) {
             const R beta = (R)1 / ((R)2 * tau_local);
             const R inv_beta = (R)2 * tau_local;
@@ -199,7 +262,7 @@ __device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R c
             return;
         }
     }
-    const R inv_tau = LES ? (R)1 / tau_local : inv_tau_base;
+    const R inv_tau = LES != LES_OFF ? (R)1 / tau_local : inv_tau_base;
 #pragma unroll
     for (int q = 0; q < Q; ++q) f[q] = f[q] - inv_tau * (f[q] - feq<L, R, DDF>(q, rho, u, uu));
 }

@@ -17,7 +17,7 @@ def make_desc(grid: LBM_Grid, config: LBM_Config) -> _lib.LbmDesc:
     return _lib.LbmDesc(
         D=grid.D, Q=grid.Q, nx=grid.nx, ny=grid.ny, nz=grid.nz, tx=tx, ty=ty, tz=tz,
         method=config.lbm_method, collision=Collisions.ids[config.collision_op], storage=config.storage_id,
-        ddf_shift=int(config.DDF_shift), les=int(config.LES), include_moving_boundary=int(config.include_moving_boundary),
+        ddf_shift=int(config.DDF_shift), les=config.les_id, include_moving_boundary=int(config.include_moving_boundary),
         included_bcs=config.included_bcs_mask, smagorinsky_cs=config.Cs,
     )
 

@@ -307,6 +307,39 @@ template <class T> struct Cell {
 };
 
 /* ---------------------------------------------------------------- one node */
+/* Learned LES closure (no reference counterpart; it replaces get_Smagorinsky_LES_tau at
+ * src/lbm/kernels/steps/collide.mojo:70-73): an MLP 9 -> 32 -> 32 -> 1 with SiLU hidden units
+ * and a softplus output times nu_scale gives nu_t from the raw features x (Pi_neq in stress-pair
+ * order, rho, |u|, Smagorinsky nu_t), normalised by (x - mean) * inv_std. Evaluated here in FP64
+ * from the float features; a feature outside [lo, hi] selects the Smagorinsky value. Parameter
+ * block: W1 (9x32, k-major), b1, W2 (32x32, k-major), b2, w3, b3, mean, inv_std, lo, hi, nu_scale. */
+const float* g_mlp = nullptr;
+
+bool mlp_nu(const float* P, const float x[9], double* nu) {
+    const int W1 = 0, B1 = 288, W2 = 320, B2 = 1344, W3 = 1376, B3 = 1408, MEAN = 1409, ISTD = 1418, LO = 1427,
+              HI = 1436, NUS = 1445;
+    double z[9];
+    for (int k = 0; k < 9; ++k) {
+        if (!(x[k] >= P[LO + k] && x[k] <= P[HI + k])) return false;
+        z[k] = ((double)x[k] - P[MEAN + k]) * P[ISTD + k];
+    }
+    double h1[32];
+    for (int j = 0; j < 32; ++j) {
+        double a = P[B1 + j];
+        for (int k = 0; k < 9; ++k) a += (double)P[W1 + k * 32 + j] * z[k];
+        h1[j] = a / (1.0 + std::exp(-a));
+    }
+    double out = P[B3];
+    for (int j = 0; j < 32; ++j) {
+        double a = P[B2 + j];
+        for (int k = 0; k < 32; ++k) a += (double)P[W2 + k * 32 + j] * h1[k];
+        out += (double)P[W3 + j] * (a / (1.0 + std::exp(-a)));
+    }
+    const double sp = out > 20.0 ? out : std::log1p(std::exp(out));
+    *nu = sp * P[NUS];
+    return true;
+}
+
 template <class S, class T> struct Stepper {
     const lbm_desc& d;
     Lat L;
@@ -389,7 +422,10 @@ template <class S, class T> struct Stepper {
      *              x + c_q is SOLID),
      *   qd >  1/2: b = f*(x, q),           f_q = (0.5 / qd) a + ((2 qd - 1) / (2 qd)) b,
      * and the link adds the momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF shifting) to the
-     * force on the solid. Returns a bit mask of the rebuilt directions (no moving-wall term there). */
+     * force on the solid. The mass the interpolation adds, dm = sum (f_q - a), is taken from the rest
+     * population (the reference's mass fix, nodewise_interpolated_BB.mojo:170-176, adds dm; subtracting
+     * it is what leaves the node the mass halfway bounce-back gives it). Returns a bit mask of the
+     * rebuilt directions (no moving-wall term there). */
     const uint32_t* link_nodes = nullptr;
     const float* link_q = nullptr;
     uint32_t n_links = 0;
@@ -406,6 +442,7 @@ template <class S, class T> struct Stepper {
         }
         if (lo >= n_links || link_nodes[lo] != node) return 0;
         uint32_t done = 0;
+        T dm = 0;  // mass the interpolation adds, taken back from the rest population
         for (int q = 1; q < L.Q; ++q) {
             const T qd = (T)link_q[(size_t)lo * (L.Q - 1) + (q - 1)];
             if (!(qd >= (T)0) || pf[q] != LBM_FLAG_SOLID) continue;
@@ -426,12 +463,14 @@ template <class S, class T> struct Stepper {
                 fb = (T)0.5 / qd * a + ((T)2 * qd - (T)1) / ((T)2 * qd) * b;
             }
             fv[q] = fb;
+            dm = dm + (fb - a);
             done |= 1u << q;
             T s = a + fb;
             if (d.ddf_shift) s = s + (T)2 * cell.w[q];
             for (int dd = 0; dd < L.D; ++dd)
                 if (L.c[q][dd]) force[dd] += (double)(L.c[q][dd] > 0 ? -s : s);
         }
+        if (done) fv[0] = fv[0] - dm;
         return done;
     }
 
@@ -491,13 +530,23 @@ template <class S, class T> struct Stepper {
         T u[3];
         cell.velocity(fv, rho, u);
         T tau_local = tau;
-        bool need_neq = d.les == LBM_LES_SMAGORINSKY || d.collision == LBM_COLL_KBC;
+        bool need_neq = d.les != LBM_LES_NONE || d.collision == LBM_COLL_KBC;
         T fneq[27], pi[6];
         if (need_neq) {
             T uu = cell.dot_uu(u);
             for (int q = 0; q < L.Q; ++q) fneq[q] = fv[q] - cell.feq(q, rho, u, uu, ddf);
             cell.pi_neq(fneq, pi);
             if (d.les == LBM_LES_SMAGORINSKY) tau_local += cell.smagorinsky_tau(pi, rho, tau, Cs);
+            if (d.les == LBM_LES_MLP) {
+                const T tau_s = cell.smagorinsky_tau(pi, rho, tau, Cs);
+                float x[9];
+                for (int n = 0; n < 6; ++n) x[n] = n < cell.nstress() ? (float)pi[n] : 0.0f;
+                x[6] = (float)rho;
+                x[7] = (float)std::sqrt(uu);
+                x[8] = (float)(tau_s / (T)3);
+                double nu = 0;
+                tau_local += (g_mlp && mlp_nu(g_mlp, x, &nu)) ? (T)(3.0 * nu) : tau_s;
+            }
         }
         if (d.collision == LBM_COLL_KBC) {
             T beta = (T)1 / ((T)2 * tau_local);
@@ -684,6 +733,9 @@ int check(const lbm_desc* d) {
 }  // namespace
 
 extern "C" {
+
+/* Parameter block of the learned closure (1446 floats, kept by the caller) for les == 2. */
+void orc_set_mlp(const float* params) { g_mlp = params; }
 
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 int orc_max_threads(void) {

@@ -42,6 +42,7 @@ def lib():
         L.orc_addr.restype = C.c_size_t
         L.orc_step_db.argtypes = [pd, vp, vp, vp, vp, d]
         L.orc_step_ep.argtypes = [pd, vp, vp, vp, d, i]
+        L.orc_set_mlp.argtypes = [vp]
         L.orc_step_db_links.argtypes = [pd, vp, vp, vp, vp, d, vp, vp, C.c_uint32, vp]
         L.orc_step_db_links.restype = i
         L.orc_macroscopic_links.argtypes = [pd, vp, vp, vp, vp, vp, i, vp, vp, C.c_uint32]
@@ -114,6 +115,17 @@ def lattice(D: int, Q: int):
 
 def addr(desc: LbmDesc, x: int, y: int, z: int, k: int, K: int) -> int:
     return lib().orc_addr(C.byref(desc), x, y, z, k, K)
+
+
+_mlp_keep = None
+
+
+def set_mlp(params) -> None:
+    """Parameter block (1446 floats) of the learned closure used when desc.les == 2."""
+    global _mlp_keep
+    _mlp_keep = np.ascontiguousarray(params, np.float32)
+    assert _mlp_keep.size == 1446
+    lib().orc_set_mlp(_p(_mlp_keep))
 
 
 def step_db(desc, f_out, f_in, bc, flags, tau):

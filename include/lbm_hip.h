@@ -230,6 +230,31 @@ int lbm_mlp_adam(float* params_dev, float* m_dev, float* v_dev, const float* gra
                  float lr, float beta1, float beta2, float eps, void* stream);
 
 /*
+ * Training data of the closure (device arrays; no reference counterpart).
+ *   lbm_box_filter:  coarse cell (I,J,K) of a cx x cy x cz grid averages the fine nodes
+ *                    factor*I + offset + a (0 <= a < factor per axis, nodes outside the fine
+ *                    grid skipped) of rho (row-major (x,y,z)) and u (row-major (x,y,z,d)) in the
+ *                    grid float type of `storage`, writing 10 floats per cell, row-major
+ *                    (I,J,K,c): rho, u_x, u_y, u_z, then <u_i u_j> for xx, xy, xz, yy, yz, zz.
+ *   lbm_les_samples: for the listed coarse cells (row-major ids; cells within 2 of the
+ *                    border get y = -1), the eddy viscosity that maps the filtered strain
+ *                    onto the subgrid stress by least squares over the 3x3x3 neighbourhood
+ *                    (clipped at 0) into y, and into X (n x 9) the features collide computes
+ *                    when the coarse LBM runs with that eddy viscosity: Pi_neq =
+ *                    -2 rho cs^2 (tau0 + 3 nu) S in stress-pair order, rho, |u| and the
+ *                    Smagorinsky nu_t of that Pi_neq with Smagorinsky constant cs.
+ *   lbm_init_fields: f = f_eq(rho(x), u(x)) per node (double buffer; rho, u laid out as
+ *                    lbm_macroscopic writes them with LBM_U_XYZD; initialize_f_from_func
+ *                    without the gradient term, src/lbm/preprocess/initial_condition.mojo:86-178).
+ */
+int lbm_box_filter(const void* rho, const void* u, int storage, int32_t nx, int32_t ny, int32_t nz,
+                   int32_t factor, int32_t offset, int32_t cx, int32_t cy, int32_t cz, float* out,
+                   void* stream);
+int lbm_les_samples(const float* coarse, int32_t cx, int32_t cy, int32_t cz, double tau0, double cs,
+                    const uint32_t* cells, int64_t n, float* X, float* y, void* stream);
+int lbm_init_fields(lbm_handle* h, void* f, const void* rho, const void* u, void* stream);
+
+/*
  * Launch tuning of one handle (no reference counterpart: the reference fixes
  * the launch shape at compile time, src/lbm/grid.mojo:99-100).
  *   LBM_OPT_XCD_REMAP: 1 = give each of the 8 XCDs a contiguous run of tiles

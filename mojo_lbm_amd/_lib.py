@@ -65,6 +65,10 @@ SIGNATURES = {
     "lbm_mlp_workspace_bytes": ([C.c_int64], _sz),
     "lbm_mlp_loss_grad": ([_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp], _i),
     "lbm_mlp_adam": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, _vp], _i),
+    "lbm_box_filter": ([_vp, _vp, _i, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                        C.c_int32, C.c_int32, _vp, _vp], _i),
+    "lbm_les_samples": ([_vp, C.c_int32, C.c_int32, C.c_int32, _d, _d, _vp, C.c_int64, _vp, _vp, _vp], _i),
+    "lbm_init_fields": ([_vp, _vp, _vp, _vp, _vp], _i),
     "lbm_link_force": ([_vp, _vp, _vp], _i),
     "lbm_f16c_convert": ([_vp, _vp, _sz, _i, _vp], _i),
     "lbm_set_option": ([_vp, _i, _i], _i),

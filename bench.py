@@ -319,6 +319,32 @@ def cylinder_batch(steps, autotune_steps=100, seed=2024, compare=False):
     return out
 
 
+def les_step_times(steps=500):
+    """128^3 D3Q19 lid-driven cube (Re = 10,000, FP32): step time with the Smagorinsky closure and with
+    the learned closure (the trained parameters shipped in mojo_lbm_amd/data), HIP events."""
+    import numpy as np
+    import torch
+
+    from mojo_lbm_amd import _lib, les
+
+    params = np.asarray(json.loads((ROOT / "mojo_lbm_amd" / "data" / "les_closure_d3q19_re10000.json")
+                                   .read_text())["params"], np.float32)
+    out = {}
+    for name, les_id, p in (("none", 0, None), ("smagorinsky", 1, None), ("learned", 2, params)):
+        c = les.Cavity(128, 0, les.tau_for(128), storage=_lib.STORE_F32, les=les_id, seed=3, mlp=p)
+        c.step(20)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        c.step(steps)
+        end.record()
+        torch.cuda.synchronize()
+        out[name] = round(start.elapsed_time(end) / steps, 5)
+        del c
+    return {"workload": "D3Q19 lid-driven cube 128^3 FP32, Re 10,000, double buffer", "ms_per_step": out,
+            "learned_over_smagorinsky": round(out["learned"] / out["smagorinsky"], 3)}
+
+
 def load_traffic(name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a profile summary
     for this kernel and workload is committed under profiles/."""
@@ -339,7 +365,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default=None,
-                    help="run one variant: fp32 | f16c | f16c_ep | fp32_ep | fp32_periodic | f16c_periodic | cylinders")
+                    help="run one variant: fp32 | f16c | f16c_ep | fp32_ep | fp32_periodic | f16c_periodic | cylinders | les")
     ap.add_argument("--xcd-remap", type=int, default=None, help="override LBM_OPT_XCD_REMAP (0 or 1)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain 1 GiB -> 1 GiB device copy kernel (for scaling rocprofv3 byte counters)")
@@ -350,6 +376,9 @@ def main():
     ap.add_argument("--cylinder-compare", action="store_true",
                     help="run only the batched cylinder cases, then every case alone, and compare")
     args = ap.parse_args()
+    if args.only == "les":
+        print(json.dumps(les_step_times()), flush=True)
+        return
     if args.cylinder_compare or args.only == "cylinders":
         print(json.dumps(cylinder_batch(args.cylinders or 20000, compare=args.cylinder_compare)), flush=True)
         return
@@ -462,6 +491,10 @@ def main():
         print(f"# cylinder batch: {cyl['mlups']:.0f} MLUPs over {cyl['cases']} cases, tile {cyl['tile']}, "
               f"grid {cyl['grid_blocks']}, {cyl['roofline_frac']:.1%} of 8 TB/s", file=sys.stderr, flush=True)
         line["cylinder_batch"] = cyl
+    if not args.only:
+        les_t = les_step_times()
+        print(f"# les 128^3: {les_t['ms_per_step']}", file=sys.stderr, flush=True)
+        line["les_closure"] = les_t
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(desc_for(0, 0, 0), tau)
     else:

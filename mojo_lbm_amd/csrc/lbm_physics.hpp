@@ -178,8 +178,22 @@ constexpr int MLP_MEAN = MLP_NPARAM, MLP_ISTD = MLP_MEAN + MLP_IN, MLP_LO = MLP_
 __host__ __device__ __forceinline__ float silu(float x) { return x / (1.0f + expf(-x)); }
 __host__ __device__ __forceinline__ float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
 
+// Inference forms used inside collide: fused multiply-adds and the hardware exp2 / log2 /
+// reciprocal (about 1 ulp each); the closure's results are compared with its FP64
+// restatement to a tolerance, not bit for bit.
+__device__ __forceinline__ float silu_fast(float x) {
+    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * x));
+}
+__device__ __forceinline__ float softplus_fast(float x) {
+    return x > 20.0f ? x : 0.6931471805599453f * __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(1.4426950408889634f * x));
+}
+
 // nu_t of the closure for raw features x; returns false when x is outside the training range.
-__device__ __forceinline__ bool mlp_nu(const float* __restrict__ P, const float (&x)[MLP_IN], float& nu) {
+// The parameter block is read through the constant address space, so every weight (the
+// same for all lanes) comes in with scalar loads and feeds the FMAs as a scalar operand.
+typedef const float __attribute__((address_space(4)))* MlpConstPtr;
+__device__ __forceinline__ bool mlp_nu(const float* __restrict__ Pg, const float (&x)[MLP_IN], float& nu) {
+    const MlpConstPtr P = (MlpConstPtr)Pg;
     float z[MLP_IN];
     bool inside = true;
 #pragma unroll
@@ -193,18 +207,18 @@ __device__ __forceinline__ bool mlp_nu(const float* __restrict__ P, const float 
     for (int j = 0; j < MLP_H; ++j) {
         float a = P[MLP_B1 + j];
 #pragma unroll
-        for (int k = 0; k < MLP_IN; ++k) a = a + P[MLP_W1 + k * MLP_H + j] * z[k];
-        h1[j] = silu(a);
+        for (int k = 0; k < MLP_IN; ++k) a = __builtin_fmaf(P[MLP_W1 + k * MLP_H + j], z[k], a);
+        h1[j] = silu_fast(a);
     }
     float out = P[MLP_B3];
 #pragma unroll 4
     for (int j = 0; j < MLP_H; ++j) {
         float a = P[MLP_B2 + j];
 #pragma unroll
-        for (int k = 0; k < MLP_H; ++k) a = a + P[MLP_W2 + k * MLP_H + j] * h1[k];
-        out = out + P[MLP_W3 + j] * silu(a);
+        for (int k = 0; k < MLP_H; ++k) a = __builtin_fmaf(P[MLP_W2 + k * MLP_H + j], h1[k], a);
+        out = __builtin_fmaf(P[MLP_W3 + j], silu_fast(a), out);
     }
-    nu = softplus(out) * P[MLP_NUSCALE];
+    nu = softplus_fast(out) * P[MLP_NUSCALE];
     return true;
 }
 

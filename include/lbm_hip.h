@@ -223,6 +223,9 @@ int lbm_set_mlp(lbm_handle* h, const float* params, int32_t n);
  *                      moment buffers m, v (1409 floats each, zero before t = 1).
  */
 int lbm_mlp_forward(const float* params_dev, const float* X_dev, int64_t n, float* nu_dev, void* stream);
+/* The closure of handle h (lbm_set_mlp) exactly as collide evaluates it (bf16 matrix cores): nu[s]
+ * for raw features X[s], or -1 where a feature lies outside the training range. */
+int lbm_mlp_closure(lbm_handle* h, const float* X_dev, int64_t n, float* nu_dev, void* stream);
 size_t lbm_mlp_workspace_bytes(int64_t n);
 int lbm_mlp_loss_grad(const float* params_dev, const float* X_dev, const float* y_dev, int64_t n,
                       float* grad_dev, double* loss_dev, void* workspace_dev, void* stream);

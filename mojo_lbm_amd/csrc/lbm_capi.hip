@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "lbm_launch.hpp"
 
@@ -345,10 +346,30 @@ int lbm_set_mlp(lbm_handle* hh, const float* params, int32_t n) {
     if (!params || n != MLP_TOTAL) return set_error(LBM_ERR_INVALID, "the closure takes 1446 parameters");
     for (int32_t i = 0; i < n; ++i)
         if (!std::isfinite(params[i])) return set_error(LBM_ERR_INVALID, "non-finite closure parameter");
-    if (!h.mlp && hipMalloc((void**)&h.mlp, MLP_TOTAL * sizeof(float)) != hipSuccess)
+    // the float block, then W1^T and W2^T as bf16 MFMA A operands (lane l: row l mod 32,
+    // K elements 8 (l / 32) + i; for W2 the K index of step s is hidden unit mfma_row(g, 8 s + i))
+    std::vector<uint8_t> blob(MLP_DEVICE_BYTES, 0);
+    std::memcpy(blob.data(), params, MLP_TOTAL * sizeof(float));
+    auto bf16 = [](float v) -> uint16_t {  // round to nearest even
+        uint32_t b;
+        std::memcpy(&b, &v, 4);
+        return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+    };
+    uint16_t* a1 = (uint16_t*)(blob.data() + MLP_PACK_A1);
+    uint16_t* a2 = (uint16_t*)(blob.data() + MLP_PACK_A2);
+    for (int l = 0; l < 64; ++l)
+        for (int i = 0; i < 8; ++i) {
+            const int g = l / 32, m = l % 32, k = 8 * g + i;
+            a1[l * 8 + i] = k < MLP_IN ? bf16(params[MLP_W1 + k * MLP_H + m]) : 0;
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const int kin = mfma_row(g, 8 * s2 + i);
+                a2[s2 * 512 + l * 8 + i] = bf16(params[MLP_W2 + kin * MLP_H + m]);
+            }
+        }
+    if (!h.mlp && hipMalloc((void**)&h.mlp, MLP_DEVICE_BYTES) != hipSuccess)
         return set_error(LBM_ERR_HIP, "hipMalloc of the closure parameters failed");
     if (hipDeviceSynchronize() != hipSuccess) return set_error(LBM_ERR_HIP, "synchronize before replacing the closure");
-    return hip_rc(hipMemcpy(h.mlp, params, MLP_TOTAL * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy");
+    return hip_rc(hipMemcpy(h.mlp, blob.data(), MLP_DEVICE_BYTES, hipMemcpyHostToDevice), "hipMemcpy");
 }
 
 int lbm_link_force(lbm_handle* hh, double* out_dev, void* stream) {

@@ -592,8 +592,9 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
         t.commit_flags(p.g, sfl);
         if (t.nonfluid) {
             if (!node_valid(t.own)) {
-                if (!links) return;
-                active = false;  // stays for the force reduction below
+                // stays for the force reduction, or for the learned closure, which needs every lane
+                if (!links && LES != LES_MLP) return;
+                active = false;
             } else {
                 if (MODE == MODE_DB) halfway_bounce_back<L, S, BIG>(t, sfl, fin, fv);
                 uint32_t rebuilt = 0;
@@ -604,7 +605,10 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
             }
         }
     }
-    if (active) {
+    if (LES == LES_MLP) {
+        collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs, p.mlp);  // whole wave (matrix cores)
+        if (active) store_populations<L, S, MODE, BIG>(t, fout, fv);
+    } else if (active) {
         collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs, p.mlp);
         store_populations<L, S, MODE, BIG>(t, fout, fv);
     }

@@ -183,6 +183,21 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(float* __restrict__ P, fl
     P[i] = P[i] - lr * mh / (sqrtf(vh) + eps);
 }
 
+// The closure exactly as collide evaluates it (wave-level, matrix cores): nu[s] for raw features
+// X[s], or -1 where a feature is outside the training range. Lanes past n repeat sample n-1 so
+// every wave stays whole.
+__global__ void __launch_bounds__(256) mlp_closure_kernel(const float* __restrict__ blob, const float* __restrict__ X,
+                                                          int64_t n, float* __restrict__ nu) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t sc = s < n ? s : n - 1;
+    float x[MLP_IN];
+#pragma unroll
+    for (int k = 0; k < MLP_IN; ++k) x[k] = X[sc * MLP_IN + k];
+    bool inside;
+    const float v = mlp_nu_wave(blob, x, inside);
+    if (s < n) nu[s] = inside ? v : -1.0f;
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -216,6 +231,15 @@ int lbm_mlp_loss_grad(const float* params_dev, const float* X_dev, const float* 
     hipLaunchKernelGGL(mlp_grad_finish, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
                        loss_part, blocks, n, grad_dev, loss_dev);
     return check_launch("mlp_grad_finish");
+}
+
+int lbm_mlp_closure(lbm_handle* hh, const float* X_dev, int64_t n, float* nu_dev, void* stream) {
+    if (!hh || !X_dev || !nu_dev || n <= 0) return set_error(LBM_ERR_INVALID, "bad closure arguments");
+    const Handle& h = *reinterpret_cast<Handle*>(hh);
+    if (!h.mlp) return set_error(LBM_ERR_INVALID, "lbm_set_mlp first");
+    hipLaunchKernelGGL(mlp_closure_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float*)h.mlp, X_dev, n, nu_dev);
+    return check_launch("mlp_closure_kernel");
 }
 
 int lbm_mlp_adam(float* params_dev, float* m_dev, float* v_dev, const float* grad_dev, int32_t t, float lr,

@@ -188,38 +188,104 @@ __device__ __forceinline__ float softplus_fast(float x) {
     return x > 20.0f ? x : 0.6931471805599453f * __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(1.4426950408889634f * x));
 }
 
-// nu_t of the closure for raw features x; returns false when x is outside the training range.
-// The parameter block is read through the constant address space, so every weight (the
-// same for all lanes) comes in with scalar loads and feeds the FMAs as a scalar operand.
+// Device copy of the parameter block: the 1446 floats, then the two hidden layers' weights
+// as bf16 MFMA A operands (v_mfma_f32_32x32x16_bf16, 8 values per lane): W1^T (32 x 16, the 9
+// inputs zero-padded) and W2^T as two 32 x 16 K-steps. Built by lbm_set_mlp.
+constexpr int MLP_PACK_A1 = 5888;                       // bytes
+constexpr int MLP_PACK_A2 = MLP_PACK_A1 + 64 * 16;      // two steps of 64 lanes x 16 bytes
+constexpr int MLP_DEVICE_BYTES = MLP_PACK_A2 + 2 * 64 * 16;
+
+// Row of the 32 x 32 MFMA result held in register j of a lane in wave half g: the result
+// column is the lane index mod 32, rows 8 (j / 4) + 4 g + j % 4.
+__host__ __device__ constexpr int mfma_row(int g, int j) { return 8 * (j / 4) + 4 * g + (j % 4); }
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+// The closure's nu_t for this lane's node, evaluated by the whole wave: the nodes of lanes
+// 0-31 and 32-63 form two 32-column blocks; each hidden layer is W^T (32 x K) times the
+// block's inputs (K x 32) on the matrix cores in bf16 with FP32 accumulation. Every lane of
+// the wave must be active. `inside` is false when a raw feature lies outside the training range.
 typedef const float __attribute__((address_space(4)))* MlpConstPtr;
-__device__ __forceinline__ bool mlp_nu(const float* __restrict__ Pg, const float (&x)[MLP_IN], float& nu) {
+
+// P[i] for this lane's wave half: both candidates are loaded at uniform addresses and the choice
+// is a vector select (readfirstlane keeps the compiler from turning the select of two loads
+// into one load at a lane-dependent address in the constant address space).
+__device__ __forceinline__ float mlp_half(MlpConstPtr P, int i_lo, int i_hi, bool hi) {
+    const float lo = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, P[i_lo])));
+    const float hv = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, P[i_hi])));
+    return hi ? hv : lo;
+}
+
+__device__ __forceinline__ float mlp_nu_wave(const float* __restrict__ Pg, const float (&x)[MLP_IN], bool& inside) {
     const MlpConstPtr P = (MlpConstPtr)Pg;
+    const int lane = (int)(threadIdx.x & 63u);
+    const bool hi = lane >= 32;
     float z[MLP_IN];
-    bool inside = true;
+    inside = true;
 #pragma unroll
     for (int k = 0; k < MLP_IN; ++k) {
-        inside = inside && x[k] >= P[MLP_LO + k] && x[k] <= P[MLP_HI + k];
+        inside = inside & (x[k] >= P[MLP_LO + k]) & (x[k] <= P[MLP_HI + k]);
         z[k] = (x[k] - P[MLP_MEAN + k]) * P[MLP_ISTD + k];
     }
-    if (!inside) return false;
-    float h1[MLP_H];
+    // B operands of layer 1: lanes of half g carry inputs 8 g .. 8 g + 7 of node lane mod 32 of
+    // the block. Block 0 (nodes 0-31) takes the low inputs from lanes 0-31 and the high inputs of
+    // the same nodes from their partner lanes l ^ 32; block 1 (nodes 32-63) the other way round:
+    // one exchange between partner lanes per register.
+    bf16x8_t lo8, hi8;
 #pragma unroll
-    for (int j = 0; j < MLP_H; ++j) {
-        float a = P[MLP_B1 + j];
-#pragma unroll
-        for (int k = 0; k < MLP_IN; ++k) a = __builtin_fmaf(P[MLP_W1 + k * MLP_H + j], z[k], a);
-        h1[j] = silu_fast(a);
+    for (int i = 0; i < 8; ++i) {
+        lo8[i] = (__bf16)z[i];
+        hi8[i] = (__bf16)(i == 0 ? z[8] : 0.0f);
     }
-    float out = P[MLP_B3];
-#pragma unroll 4
-    for (int j = 0; j < MLP_H; ++j) {
-        float a = P[MLP_B2 + j];
+    const u32x4_t ulo = __builtin_bit_cast(u32x4_t, lo8), uhi = __builtin_bit_cast(u32x4_t, hi8);
+    u32x4_t b0, b1;
 #pragma unroll
-        for (int k = 0; k < MLP_H; ++k) a = __builtin_fmaf(P[MLP_W2 + k * MLP_H + j], h1[k], a);
-        out = __builtin_fmaf(P[MLP_W3 + j], silu_fast(a), out);
+    for (int d = 0; d < 4; ++d) {
+        const uint32_t t = (uint32_t)__shfl_xor((int)(hi ? ulo[d] : uhi[d]), 32, 64);
+        b0[d] = hi ? t : ulo[d];
+        b1[d] = hi ? uhi[d] : t;
     }
-    nu = softplus_fast(out) * P[MLP_NUSCALE];
-    return true;
+    const uint8_t* pack = (const uint8_t*)Pg;
+    const bf16x8_t a1 = *(const bf16x8_t*)(pack + MLP_PACK_A1 + lane * 16);
+    const bf16x8_t a20 = *(const bf16x8_t*)(pack + MLP_PACK_A2 + lane * 16);
+    const bf16x8_t a21 = *(const bf16x8_t*)(pack + MLP_PACK_A2 + 1024 + lane * 16);
+    // one 32-node block at a time keeps a single block's accumulators live
+    float part[2];
+#pragma unroll 1
+    for (int blk = 0; blk < 2; ++blk) {
+        const u32x4_t bin = blk ? b1 : b0;
+        f32x16_t c;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) c[j] = mlp_half(P, MLP_B1 + mfma_row(0, j), MLP_B1 + mfma_row(1, j), hi);
+        const f32x16_t d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, __builtin_bit_cast(bf16x8_t, bin), c, 0, 0, 0);
+        // layer 2: the K index of step s, half g, element i is hidden unit mfma_row(g, 8 s + i),
+        // which is exactly where layer 1 left it; W2 is packed to match
+        bf16x8_t h0, h1;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            h0[i] = (__bf16)silu_fast(d1[i]);
+            h1[i] = (__bf16)silu_fast(d1[8 + i]);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) c[j] = mlp_half(P, MLP_B2 + mfma_row(0, j), MLP_B2 + mfma_row(1, j), hi);
+        f32x16_t d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a20, h0, c, 0, 0, 0);
+        d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a21, h1, d2, 0, 0, 0);
+        // layer 3: partial dot product over this lane's 16 rows
+        float sacc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const float w = mlp_half(P, MLP_W3 + mfma_row(0, j), MLP_W3 + mfma_row(1, j), hi);
+            sacc = __builtin_fmaf(w, silu_fast(d2[j]), sacc);
+        }
+        part[blk] = sacc;
+    }
+    const float sa = part[0], sb = part[1];
+    // column l of block 0 is on lanes l and l + 32, column l of block 1 on lanes l and l + 32 too
+    const float t = __shfl_xor(hi ? sa : sb, 32, 64);
+    const float out = P[MLP_B3] + ((hi ? sb : sa) + t);
+    return softplus_fast(out) * P[MLP_NUSCALE];
 }
 
 // collide, src/lbm/kernels/steps/collide.mojo:23-90 (SRT: ops/collisions/collision.mojo:12-46,
@@ -251,8 +317,9 @@ __device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R c
             x[6] = (float)rho;
             x[7] = (float)sqrt(uu);
             x[8] = (float)(tau_s / (R)3);
-            float nu;
-            tau_local = mlp_nu(mlp, x, nu) ? tau_local + (R)3 * (R)nu : tau_local + tau_s;
+            bool inside;
+            const float nu = mlp_nu_wave(mlp, x, inside);
+            tau_local = inside ? tau_local + (R)3 * (R)nu : tau_local + tau_s;
         }
         if (COLL == C_KBC) {
             const R beta = (R)1 / ((R)2 * tau_local);

@@ -311,9 +311,22 @@ template <class T> struct Cell {
  * src/lbm/kernels/steps/collide.mojo:70-73): an MLP 9 -> 32 -> 32 -> 1 with SiLU hidden units
  * and a softplus output times nu_scale gives nu_t from the raw features x (Pi_neq in stress-pair
  * order, rho, |u|, Smagorinsky nu_t), normalised by (x - mean) * inv_std. Evaluated here in FP64
- * from the float features; a feature outside [lo, hi] selects the Smagorinsky value. Parameter
+ * from the float features, with the two hidden layers' inputs and weights rounded to bf16 as the
+ * library's matrix-core evaluation takes them; a feature outside [lo, hi] selects the Smagorinsky
+ * value. Parameter
  * block: W1 (9x32, k-major), b1, W2 (32x32, k-major), b2, w3, b3, mean, inv_std, lo, hi, nu_scale. */
 const float* g_mlp = nullptr;
+
+/* bf16 rounding (nearest even) of a float: the precision of the hidden layers' inputs and weights
+ * when the library evaluates them on the matrix cores. */
+double bf16r(float v) {
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &b, 4);
+    return r;
+}
 
 bool mlp_nu(const float* P, const float x[9], double* nu) {
     const int W1 = 0, B1 = 288, W2 = 320, B2 = 1344, W3 = 1376, B3 = 1408, MEAN = 1409, ISTD = 1418, LO = 1427,
@@ -321,18 +334,18 @@ bool mlp_nu(const float* P, const float x[9], double* nu) {
     double z[9];
     for (int k = 0; k < 9; ++k) {
         if (!(x[k] >= P[LO + k] && x[k] <= P[HI + k])) return false;
-        z[k] = ((double)x[k] - P[MEAN + k]) * P[ISTD + k];
+        z[k] = bf16r((x[k] - P[MEAN + k]) * P[ISTD + k]);
     }
     double h1[32];
     for (int j = 0; j < 32; ++j) {
         double a = P[B1 + j];
-        for (int k = 0; k < 9; ++k) a += (double)P[W1 + k * 32 + j] * z[k];
-        h1[j] = a / (1.0 + std::exp(-a));
+        for (int k = 0; k < 9; ++k) a += bf16r(P[W1 + k * 32 + j]) * z[k];
+        h1[j] = bf16r((float)(a / (1.0 + std::exp(-a))));
     }
     double out = P[B3];
     for (int j = 0; j < 32; ++j) {
         double a = P[B2 + j];
-        for (int k = 0; k < 32; ++k) a += (double)P[W2 + k * 32 + j] * h1[k];
+        for (int k = 0; k < 32; ++k) a += bf16r(P[W2 + k * 32 + j]) * h1[k];
         out += (double)P[W3 + j] * (a / (1.0 + std::exp(-a)));
     }
     const double sp = out > 20.0 ? out : std::log1p(std::exp(out));
@@ -736,6 +749,14 @@ extern "C" {
 
 /* Parameter block of the learned closure (1446 floats, kept by the caller) for les == 2. */
 void orc_set_mlp(const float* params) { g_mlp = params; }
+
+/* The closure as the oracle's collide evaluates it: nu[s] for raw features X[s] (n x 9), -1 outside. */
+void orc_mlp_closure(const float* params, const float* X, long n, double* nu) {
+    for (long s = 0; s < n; ++s) {
+        double v = 0;
+        nu[s] = mlp_nu(params, X + 9 * s, &v) ? v : -1.0;
+    }
+}
 
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 int orc_max_threads(void) {

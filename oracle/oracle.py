@@ -43,6 +43,7 @@ def lib():
         L.orc_step_db.argtypes = [pd, vp, vp, vp, vp, d]
         L.orc_step_ep.argtypes = [pd, vp, vp, vp, d, i]
         L.orc_set_mlp.argtypes = [vp]
+        L.orc_mlp_closure.argtypes = [vp, vp, C.c_long, vp]
         L.orc_step_db_links.argtypes = [pd, vp, vp, vp, vp, d, vp, vp, C.c_uint32, vp]
         L.orc_step_db_links.restype = i
         L.orc_macroscopic_links.argtypes = [pd, vp, vp, vp, vp, vp, i, vp, vp, C.c_uint32]
@@ -126,6 +127,15 @@ def set_mlp(params) -> None:
     _mlp_keep = np.ascontiguousarray(params, np.float32)
     assert _mlp_keep.size == 1446
     lib().orc_set_mlp(_p(_mlp_keep))
+
+
+def mlp_closure(params, X):
+    """nu_t of the closure for raw features X [n, 9] (-1 outside the training range)."""
+    P = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.zeros(len(X), np.float64)
+    lib().orc_mlp_closure(_p(P), _p(X), len(X), _p(out))
+    return out
 
 
 def step_db(desc, f_out, f_in, bc, flags, tau):

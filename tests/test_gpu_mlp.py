@@ -4,8 +4,10 @@ restatement (oracle/mlp_ref.py), and the closure inside collide against the orac
 Tolerances: the kernels compute in FP32 (accumulating gradient partials in FP64), the
 restatement in FP64: forward rtol 2e-5, loss rtol 1e-5, gradient within 2e-4 of its
 largest entry, Adam parameters rtol 1e-6. Populations after steps with the closure:
-max |df| <= 1e-6 (FP32 storage) / 1e-8 (FP64 storage: the closure itself runs in FP32) /
-below 2^-14 (one Float16C code step at |f| = 1/8) on at most 0.1 % of the populations.
+max |df| <= 1e-5 (the closure's hidden layers take bf16 inputs on the matrix cores, and the
+oracle rounds the same quantities to bf16, so an ulp-level difference upstream can flip a bf16
+rounding) / below 2^-14 (one Float16C code step at |f| = 1/8) on at most 0.1 % of the
+populations; the closure evaluation itself agrees with the oracle's to rtol 2e-3 (typically 1e-7).
 With a training range that excludes every node, the closure must reproduce the
 Smagorinsky run bit for bit."""
 import ctypes as C
@@ -134,6 +136,27 @@ def _closure_case(storage, method, seed, wide=True):
     return desc, f, bc, flags, P
 
 
+def test_closure_evaluation_matches_oracle(gpu):
+    # the collide-side evaluation (bf16 matrix cores) against the oracle's bf16-rounded FP64 MLP
+    import torch
+
+    desc = U.make_desc("D3Q19", (16, 16, 16), les=2)
+    h = _lib.Handle(desc)
+    P = M.random_params(41, nu_scale=1e-2)
+    P[M.LO + 4] = -1.5  # some samples fall outside the range
+    L = _lib.lib()
+    _lib.check(L.lbm_set_mlp(h.ptr, P.ctypes.data, len(P)))
+    X = np.random.default_rng(7).standard_normal((1000, 9)).astype(np.float32)
+    Xd = _dev(X)
+    nu = torch.zeros(1000, dtype=torch.float32, device="cuda")
+    _lib.check(L.lbm_mlp_closure(h.ptr, Xd.data_ptr(), 1000, nu.data_ptr(), _stream()))
+    got = nu.cpu().numpy().astype(np.float64)
+    ref = O.mlp_closure(P, X)
+    assert np.array_equal(got < 0, ref < 0)
+    inside = ref >= 0
+    assert np.allclose(got[inside], ref[inside], rtol=2e-3, atol=1e-7), np.max(np.abs(got - ref)[inside])
+
+
 @pytest.mark.parametrize("storage,method", [(0, 0), (0, 1), (1, 0), (2, 0)])
 def test_closure_in_collide_matches_oracle(gpu, storage, method):
     desc, f, bc, flags, P = _closure_case(storage, method, seed=20 + storage + 3 * method)
@@ -157,8 +180,9 @@ def test_closure_in_collide_matches_oracle(gpu, storage, method):
         n_diff = np.count_nonzero(got != ref)
         assert U.max_abs_diff(desc, got, ref) <= 2.0 ** -14 and n_diff <= 1e-3 * got.size, n_diff
         return
-    tol = 1e-8 if storage == 2 else 1e-6
-    assert U.max_abs_diff(desc, got, ref) <= tol
+    # the closure takes bf16 inputs on the matrix cores (the oracle rounds the same values to bf16):
+    # an ulp-level difference can flip one bf16 rounding and move nu_t by ~0.4 % at that node
+    assert U.max_abs_diff(desc, got, ref) <= 1e-5
 
 
 def test_closure_outside_range_is_smagorinsky(gpu):

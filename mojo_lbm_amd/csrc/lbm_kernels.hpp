@@ -632,6 +632,16 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParam
     step_tile<L, S, MODE, COLL, LES, DDF, BIG, LINKS>(p, tile, tile, sfl);
 }
 
+// The step with the learned closure, held to at most 128 VGPRs (two 512-lane workgroups per CU)
+// so the population traffic keeps enough waves in flight around the matrix-core work.
+template <class L, class S, int MODE, int COLL, bool DDF>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    step_kernel_mlp(StepParams<typename S::R> p) {
+    extern __shared__ uint8_t sfl[];
+    const uint32_t tile = (uint32_t)tile_of_block(p.g);
+    step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, tile, tile, sfl);
+}
+
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125) and the FP64 moment
 // reduction: re-stream + BC (tau = 0), moments; solid nodes (raw flag == SOLID) report bc.
 template <class L, class S, int MODE, bool DDF, bool REDUCE, bool BIG>

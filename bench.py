@@ -225,7 +225,7 @@ def accuracy_study(N, checkpoints, u_lid=0.1, Re=1000.0):
     print(json.dumps(line), flush=True)
 
 
-def cylinder_batch(steps, autotune_steps=100, seed=2024, compare=False):
+def cylinder_batch(steps, autotune_steps=100, seed=2024, compare=False, tile=None):
     """64 seeded 2-D cylinder channels (D2Q9 SRT FP32, interpolated bounce-back) advanced together:
     one launch per step for all cases, hipGraph replays. Times `steps` steps from rest after the
     autotuner has picked the tile shape and grid; returns aggregate MLUPs over the real (unpadded)
@@ -246,7 +246,11 @@ def cylinder_batch(steps, autotune_steps=100, seed=2024, compare=False):
         flags, bc, links = c.arrays(lat)
         b.set_case(i, flags, bc, links)
     t0 = time.time()
-    tile, grid = b.autotune(autotune_steps)
+    if tile is None:
+        tile, grid = b.autotune(autotune_steps)
+    else:  # a fixed tile shape (profiling runs that should hold one kernel configuration only)
+        b.set_tile(tile)
+        grid = 0
     tune_s = time.time() - t0
     info = b.info()
     b.init_equilibrium()
@@ -375,12 +379,16 @@ def main():
                     help="steps of the 64-case batched cylinder run reported under 'cylinder_batch' (0: skip)")
     ap.add_argument("--cylinder-compare", action="store_true",
                     help="run only the batched cylinder cases, then every case alone, and compare")
+    ap.add_argument("--cylinder-tile", default=None,
+                    help="fixed tile shape for the cylinder batch, e.g. 32x8x1 (default: autotune)")
     args = ap.parse_args()
     if args.only == "les":
         print(json.dumps(les_step_times()), flush=True)
         return
     if args.cylinder_compare or args.only == "cylinders":
-        print(json.dumps(cylinder_batch(args.cylinders or 20000, compare=args.cylinder_compare)), flush=True)
+        tile = tuple(int(v) for v in args.cylinder_tile.split("x")) if args.cylinder_tile else None
+        print(json.dumps(cylinder_batch(args.cylinders or 20000, compare=args.cylinder_compare, tile=tile)),
+              flush=True)
         return
     if args.accuracy:
         accuracy_study(args.n, [int(s) for s in args.accuracy.split(",")])

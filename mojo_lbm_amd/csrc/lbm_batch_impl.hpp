@@ -56,18 +56,21 @@ template <class L, class S> struct BatchLaunchImpl {
         return check_launch("batch_step_kernel");
     }
 
-    static int step(const Batch& b, int parity, hipStream_t st) {
-        const bool kbc = b.proto.collision == LBM_COLL_KBC;
+    template <int COLL>
+    static int step_c(const Batch& b, int parity, hipStream_t st) {
         const bool les = b.proto.les == LBM_LES_SMAGORINSKY;
         const bool ddf = b.proto.ddf_shift != 0;
-        if (!kbc && !les && !ddf) return step_k<C_SRT, false, false>(b, parity, st);
-        if (!kbc && !les && ddf) return step_k<C_SRT, false, true>(b, parity, st);
-        if (!kbc && les && !ddf) return step_k<C_SRT, true, false>(b, parity, st);
-        if (!kbc && les && ddf) return step_k<C_SRT, true, true>(b, parity, st);
-        if (kbc && !les && !ddf) return step_k<C_KBC, false, false>(b, parity, st);
-        if (kbc && !les && ddf) return step_k<C_KBC, false, true>(b, parity, st);
-        if (kbc && les && !ddf) return step_k<C_KBC, true, false>(b, parity, st);
-        return step_k<C_KBC, true, true>(b, parity, st);
+        if (les) return ddf ? step_k<COLL, LES_SMAG, true>(b, parity, st) : step_k<COLL, LES_SMAG, false>(b, parity, st);
+        return ddf ? step_k<COLL, LES_OFF, true>(b, parity, st) : step_k<COLL, LES_OFF, false>(b, parity, st);
+    }
+
+    static int step(const Batch& b, int parity, hipStream_t st) {
+        switch (b.proto.collision) {
+            case LBM_COLL_TRT: return step_c<C_TRT>(b, parity, st);
+            case LBM_COLL_KBC: return step_c<C_KBC>(b, parity, st);
+            case LBM_COLL_RLBM: return step_c<C_RLBM>(b, parity, st);
+            default: return step_c<C_SRT>(b, parity, st);
+        }
     }
 
     // Per-case flag summaries and the device case records.

@@ -60,9 +60,8 @@ int validate(const lbm_desc* d) {
         return set_error(LBM_ERR_INVALID, "tile too small: needs >= 32 nodes and a halo of at most 8 nodes per node");
     if (d->method != LBM_DOUBLE_BUFFER && d->method != LBM_ESOTERIC_PULL)
         return set_error(LBM_ERR_INVALID, "method must be double buffer or esoteric pull");
-    if (d->collision == LBM_COLL_TRT || d->collision == LBM_COLL_RLBM)
-        return set_error(LBM_ERR_UNSUPPORTED, "TRT and RLBM collision are not provided by this build");
-    if (d->collision != LBM_COLL_SRT && d->collision != LBM_COLL_KBC)
+    if (d->collision != LBM_COLL_SRT && d->collision != LBM_COLL_TRT && d->collision != LBM_COLL_KBC &&
+        d->collision != LBM_COLL_RLBM)
         return set_error(LBM_ERR_INVALID, "unknown collision operator");
     if (d->storage < LBM_STORE_F32 || d->storage > LBM_STORE_F64) return set_error(LBM_ERR_INVALID, "unknown storage");
     if (d->les != LBM_LES_NONE && d->les != LBM_LES_SMAGORINSKY && d->les != LBM_LES_MLP)

@@ -46,25 +46,27 @@ template <class L, class S> struct LaunchImpl {
         }
     }
 
-    template <int MODE>
-    static int step_mode(const Handle& h, const StepParams<R>& p, hipStream_t st) {
-        const bool kbc = h.d.collision == LBM_COLL_KBC;
-        const bool les = h.d.les == LBM_LES_SMAGORINSKY;
+    template <int MODE, int COLL>
+    static int step_coll(const Handle& h, const StepParams<R>& p, hipStream_t st) {
         const bool ddf = h.d.ddf_shift != 0;
         if (h.d.les == LBM_LES_MLP) {
-            if (!kbc && !ddf) return step_mlp<MODE, C_SRT, false>(h, p, st);
-            if (!kbc && ddf) return step_mlp<MODE, C_SRT, true>(h, p, st);
-            if (kbc && !ddf) return step_mlp<MODE, C_KBC, false>(h, p, st);
-            return step_mlp<MODE, C_KBC, true>(h, p, st);
+            if constexpr (COLL == C_SRT || COLL == C_KBC)
+                return ddf ? step_mlp<MODE, COLL, true>(h, p, st) : step_mlp<MODE, COLL, false>(h, p, st);
+            return set_error(LBM_ERR_UNSUPPORTED, "the learned closure is provided with SRT and KBC collision");
         }
-        if (!kbc && !les && !ddf) return step_k<MODE, C_SRT, false, false>(h, p, st);
-        if (!kbc && !les && ddf) return step_k<MODE, C_SRT, false, true>(h, p, st);
-        if (!kbc && les && !ddf) return step_k<MODE, C_SRT, true, false>(h, p, st);
-        if (!kbc && les && ddf) return step_k<MODE, C_SRT, true, true>(h, p, st);
-        if (kbc && !les && !ddf) return step_k<MODE, C_KBC, false, false>(h, p, st);
-        if (kbc && !les && ddf) return step_k<MODE, C_KBC, false, true>(h, p, st);
-        if (kbc && les && !ddf) return step_k<MODE, C_KBC, true, false>(h, p, st);
-        return step_k<MODE, C_KBC, true, true>(h, p, st);
+        if (h.d.les == LBM_LES_SMAGORINSKY)
+            return ddf ? step_k<MODE, COLL, LES_SMAG, true>(h, p, st) : step_k<MODE, COLL, LES_SMAG, false>(h, p, st);
+        return ddf ? step_k<MODE, COLL, LES_OFF, true>(h, p, st) : step_k<MODE, COLL, LES_OFF, false>(h, p, st);
+    }
+
+    template <int MODE>
+    static int step_mode(const Handle& h, const StepParams<R>& p, hipStream_t st) {
+        switch (h.d.collision) {
+            case LBM_COLL_TRT: return step_coll<MODE, C_TRT>(h, p, st);
+            case LBM_COLL_KBC: return step_coll<MODE, C_KBC>(h, p, st);
+            case LBM_COLL_RLBM: return step_coll<MODE, C_RLBM>(h, p, st);
+            default: return step_coll<MODE, C_SRT>(h, p, st);
+        }
     }
 
     static int step(const Handle& h, void* f_out, const void* f_in, const void* bc, const uint8_t* flags, double tau,

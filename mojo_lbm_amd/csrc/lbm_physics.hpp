@@ -156,7 +156,7 @@ __device__ __forceinline__ void kbc_shear(const R (&pi)[6], R (&s)[L::Q]) {
     }
 }
 
-enum Coll { C_SRT = 0, C_KBC = 2 };
+enum Coll { C_SRT = 0, C_TRT = 1, C_KBC = 2, C_RLBM = 3 };
 enum LesMode { LES_OFF = 0, LES_SMAG = 1, LES_MLP = 2 };
 
 // ------------------------------------------------------------ learned closure
@@ -302,7 +302,7 @@ __device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R c
     velocity<L, R>(f, rho, u);
     const R uu = dot_uu<L, R>(u);
     R tau_local = tau;
-    if (LES || COLL == C_KBC) {
+    if (LES || COLL == C_KBC || COLL == C_RLBM) {
         R fn[Q];
         R pi[6];
 #pragma unroll
@@ -320,6 +320,34 @@ __device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R c
             bool inside;
             const float nu = mlp_nu_wave(mlp, x, inside);
             tau_local = inside ? tau_local + (R)3 * (R)nu : tau_local + tau_s;
+        }
+        if (COLL == C_RLBM) {
+            // RLBM, ops/collisions/collision.mojo:168-223: f_q = f_eq + (1 - 1/tau) w_q/(2 cs^4) H_q : Pi_neq
+            // with H_q,ab = c_a c_b - cs^2 delta_ab (off-diagonal pairs counted twice)
+            const R inv_tau = (R)1 / tau_local;
+            const R keep = (R)1 - inv_tau;
+            const R cs2 = (R)(1.0 / 3.0);
+            constexpr int NS = L::D * (L::D + 1) / 2;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                bool first = true;
+                R dot = 0;
+#pragma unroll
+                for (int n = 0; n < NS; ++n) {
+                    const int a = stress_a(L::D, n), b = stress_b(L::D, n);
+                    const int cc = L::c(q, a) * L::c(q, b);
+                    R h;
+                    if (a == b) h = (R)cc - cs2;
+                    else if (cc == 0) continue;
+                    else h = (R)cc * (R)2;
+                    const R t = h * pi[n];
+                    dot = first ? t : dot + t;
+                    first = false;
+                }
+                const R k = weight<L, R>(q) / (R)(2.0 / 9.0);
+                f[q] = feq<L, R, DDF>(q, rho, u, uu) + keep * (k * dot);
+            }
+            return;
         }
         if (COLL == C_KBC) {
             const R beta = (R)1 / ((R)2 * tau_local);
@@ -342,6 +370,28 @@ __device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R c
             }
             return;
         }
+    }
+    if (COLL == C_TRT) {
+        // TRT, ops/collisions/collision.mojo:49-113, with tau_asymm = 1/2 + Lambda / (tau - 1/2),
+        // Lambda = 3/16 (steps/collide.mojo:83-85): symmetric and antisymmetric parts of each
+        // opposite pair (q, q+1) relax with their own rates; the rest population relaxes with tau.
+        const R tau_a = (R)0.5 + (R)0.1875 / (tau_local - (R)0.5);
+        const R inv_ts = (R)1 / tau_local;
+        const R inv_ta = (R)1 / tau_a;
+        f[0] = f[0] - inv_ts * (f[0] - feq<L, R, DDF>(0, rho, u, uu));
+#pragma unroll
+        for (int q = 1; q < Q; q += 2) {
+            const int o = q + 1;
+            const R fs = (f[q] + f[o]) * (R)0.5;
+            const R fa = (f[q] - f[o]) * (R)0.5;
+            const R eq = feq<L, R, DDF>(q, rho, u, uu);
+            const R eo = feq<L, R, DDF>(o, rho, u, uu);
+            const R es = (eq + eo) * (R)0.5;
+            const R ea = (eq - eo) * (R)0.5;
+            f[q] = f[q] - (inv_ts * (fs - es) + inv_ta * (fa - ea));
+            f[o] = f[o] - (inv_ts * (fs - es) + inv_ta * ((-fa) - (-ea)));
+        }
+        return;
     }
     const R inv_tau = LES != LES_OFF ? (R)1 / tau_local : inv_tau_base;
 #pragma unroll

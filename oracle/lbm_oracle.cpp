@@ -543,7 +543,7 @@ template <class S, class T> struct Stepper {
         T u[3];
         cell.velocity(fv, rho, u);
         T tau_local = tau;
-        bool need_neq = d.les != LBM_LES_NONE || d.collision == LBM_COLL_KBC;
+        bool need_neq = d.les != LBM_LES_NONE || d.collision == LBM_COLL_KBC || d.collision == LBM_COLL_RLBM;
         T fneq[27], pi[6];
         if (need_neq) {
             T uu = cell.dot_uu(u);
@@ -561,6 +561,34 @@ template <class S, class T> struct Stepper {
                 tau_local += (g_mlp && mlp_nu(g_mlp, x, &nu)) ? (T)(3.0 * nu) : tau_s;
             }
         }
+        if (d.collision == LBM_COLL_RLBM) {
+            /* RLBM, src/lbm/kernels/ops/collisions/collision.mojo:168-223: the non-equilibrium part is
+             * replaced by its second-order Hermite projection w_q / (2 cs^4) H_q : Pi_neq,
+             * H_q,ab = c_a c_b - cs^2 delta_ab, off-diagonal pairs counted twice (get_kbc_Qiab :138-166). */
+            T inv_tau = (T)1 / tau_local;
+            T keep = (T)1 - inv_tau;
+            T uu = cell.dot_uu(u);
+            const T cs2 = (T)(1.0 / 3.0);
+            for (int q = 0; q < L.Q; ++q) {
+                T dot = 0;
+                bool first = true;
+                for (int n = 0; n < cell.nstress(); ++n) {
+                    int a, b;
+                    cell.stress_pair(n, a, b);
+                    int cc = L.c[q][a] * L.c[q][b];
+                    T h;
+                    if (a == b) h = (T)cc - cs2;
+                    else if (cc == 0) continue;
+                    else h = (T)cc * (T)2;
+                    T t = h * pi[n];
+                    dot = first ? t : dot + t;
+                    first = false;
+                }
+                T k = cell.w[q] / (T)(2.0 / 9.0);
+                fv[q] = cell.feq(q, rho, u, uu, ddf) + keep * (k * dot);
+            }
+            return;
+        }
         if (d.collision == LBM_COLL_KBC) {
             T beta = (T)1 / ((T)2 * tau_local);
             T inv_beta = (T)2 * tau_local;
@@ -574,6 +602,23 @@ template <class S, class T> struct Stepper {
             for (int q = 0; q < L.Q; ++q) sp2 += dh[q] * dh[q] / feq[q];
             T gamma = inv_beta - ((T)2 - inv_beta) * sp1 / (sp2 + (T)1e-32);
             for (int q = 0; q < L.Q; ++q) fv[q] -= beta * (ds[q] * (T)2 + gamma * dh[q]);
+            return;
+        }
+        if (d.collision == LBM_COLL_TRT) {
+            /* TRT, src/lbm/kernels/ops/collisions/collision.mojo:49-113; tau_asymm = 1/2 + (3/16)/(tau - 1/2)
+             * (src/lbm/kernels/steps/collide.mojo:83-85). Opposite directions are the pairs (q, q+1), q odd. */
+            T uu = cell.dot_uu(u);
+            T tau_a = (T)0.5 + (T)0.1875 / (tau_local - (T)0.5);
+            T inv_ts = (T)1 / tau_local, inv_ta = (T)1 / tau_a;
+            fv[0] = fv[0] - inv_ts * (fv[0] - cell.feq(0, rho, u, uu, ddf));
+            for (int q = 1; q < L.Q; q += 2) {
+                int o = q + 1;
+                T fs = (fv[q] + fv[o]) * (T)0.5, fa = (fv[q] - fv[o]) * (T)0.5;
+                T eq = cell.feq(q, rho, u, uu, ddf), eo = cell.feq(o, rho, u, uu, ddf);
+                T es = (eq + eo) * (T)0.5, ea = (eq - eo) * (T)0.5;
+                fv[q] = fv[q] - (inv_ts * (fs - es) + inv_ta * (fa - ea));
+                fv[o] = fv[o] - (inv_ts * (fs - es) + inv_ta * ((-fa) - (-ea)));
+            }
             return;
         }
         // SRT

@@ -81,10 +81,19 @@ def test_create_rejects_invalid_descriptions(mod, msg):
     assert msg.lower() in err.lower()
 
 
-def test_trt_and_rlbm_are_reported_unsupported():
-    for coll in (_lib.COLL_TRT, _lib.COLL_RLBM):
+def test_collision_operators_validate():
+    # every operator of the reference's Collisions set passes validation (the CPU box then reports
+    # no device); an unknown id is invalid
+    for coll in (_lib.COLL_SRT, _lib.COLL_TRT, _lib.COLL_KBC, _lib.COLL_RLBM):
         rc, err = _create_rc(U.make_desc("D2Q9", (32, 32), collision=coll))
-        assert rc == _lib.ERR_UNSUPPORTED, err
+        assert rc in (_lib.OK, _lib.ERR_NO_DEVICE), err
+    rc, err = _create_rc(U.make_desc("D2Q9", (32, 32), collision=7))
+    assert rc == _lib.ERR_INVALID
+
+
+def test_learned_closure_needs_3d():
+    rc, err = _create_rc(U.make_desc("D2Q9", (32, 32), les=2))
+    assert rc == _lib.ERR_UNSUPPORTED, err
 
 
 def test_create_without_gpu_reports_no_device():

@@ -24,6 +24,10 @@ VARIANTS = {
     "srt_les": dict(collision=0, les=1, ddf=0),
     "kbc": dict(collision=2, les=0, ddf=0),
     "kbc_les_ddf": dict(collision=2, les=1, ddf=1),
+    "trt": dict(collision=1, les=0, ddf=0),
+    "trt_les_ddf": dict(collision=1, les=1, ddf=1),
+    "rlbm": dict(collision=3, les=0, ddf=0),
+    "rlbm_les_ddf": dict(collision=3, les=1, ddf=1),
 }
 TAU = 0.55  # low viscosity so LES and KBC terms are active
 
@@ -57,7 +61,7 @@ def test_double_buffer_matches_oracle(gpu, lat, store, variant):
     assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), "not bit-identical to the oracle"
 
 
-@pytest.mark.parametrize("variant", ["srt", "srt_les", "kbc"])
+@pytest.mark.parametrize("variant", ["srt", "srt_les", "kbc", "trt", "rlbm"])
 @pytest.mark.parametrize("store", list(STORES))
 @pytest.mark.parametrize("lat", list(SIZES))
 def test_esoteric_pull_matches_oracle(gpu, lat, store, variant):

@@ -37,4 +37,7 @@ def __getattr__(name):
     if name in ("calculate_rho_and_velocity", "reduce_moments"):
         from . import output
         return getattr(output, name)
+    if name == "snapshot":
+        import importlib
+        return importlib.import_module(".snapshot", __name__)
     raise AttributeError(name)

@@ -349,6 +349,47 @@ def les_step_times(steps=500):
             "learned_over_smagorinsky": round(out["learned"] / out["smagorinsky"], 3)}
 
 
+def output_times(desc, tau, u_lid, reps=20):
+    """Post-processing on the benchmark state after 100 steps: lbm_macroscopic (rho, u) and
+    lbm_q_criterion, ms per call (HIP events on the launch stream) and algorithmic GB/s, where a
+    node costs its populations + flag byte, plus its rho, u writes (macroscopic) or its own u
+    read and Q write (Q-criterion; neighbour velocities are counted as cache hits)."""
+    import torch
+
+    from mojo_lbm_amd import _lib
+
+    run = make_run(desc, u_lid)
+    time_run(run, tau, 100, 0)  # the state ends in run["f"]
+    L, h, st = run["L"], run["h"], run["st"]
+    n, D, Q = desc.nx * desc.ny * desc.nz, desc.D, desc.Q
+    rho = torch.empty(n, dtype=torch.float32, device="cuda")
+    u = torch.empty(n * D, dtype=torch.float32, device="cuda")
+    q = torch.empty(n, dtype=torch.float32, device="cuda")
+    args = (run["f"].data_ptr(), run["bc"].data_ptr(), run["flags"].data_ptr())
+    calls = {
+        "macroscopic": (lambda: L.lbm_macroscopic(h.ptr, rho.data_ptr(), u.data_ptr(), *args, 1, _lib.U_XYZD, st),
+                        n * (Q * 4 + 1 + 4 * (D + 1))),
+        "q_criterion": (lambda: L.lbm_q_criterion(h.ptr, q.data_ptr(), u.data_ptr(), *args, tau, 1, _lib.U_XYZD, st),
+                        n * (Q * 4 + 1 + 4 * D + 4)),
+    }
+    out = {}
+    for name, (fn, nbytes) in calls.items():
+        _lib.check(fn())
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(reps):
+            fn()
+        end.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(end) / reps
+        out[name] = {"ms": round(ms, 5), "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "roofline_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    del run
+    torch.cuda.empty_cache()
+    return {"workload": f"D3Q19 FP32 lid-driven cube {desc.nx}^3 after 100 steps", **out}
+
+
 def load_traffic(name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if a profile summary
     for this kernel and workload is committed under profiles/."""
@@ -500,6 +541,9 @@ def main():
               f"grid {cyl['grid_blocks']}, {cyl['roofline_frac']:.1%} of 8 TB/s", file=sys.stderr, flush=True)
         line["cylinder_batch"] = cyl
     if not args.only:
+        outs = output_times(desc_for(0, 0, 0), tau, u_lid)
+        print(f"# outputs: {outs}", file=sys.stderr, flush=True)
+        line["outputs"] = outs
         les_t = les_step_times()
         print(f"# les 128^3: {les_t['ms_per_step']}", file=sys.stderr, flush=True)
         line["les_closure"] = les_t

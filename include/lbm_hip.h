@@ -141,6 +141,20 @@ int lbm_macroscopic(lbm_handle* h, void* rho, void* u, const void* f, const void
                     const uint8_t* flags, int after_odd, int u_layout, void* stream);
 
 /*
+ * Q-criterion Q = 0.25 |omega|^2 - 0.5 |S|^2 per node (nx*ny*nz, row-major (x,y,z), grid float
+ * type): replaces calculate_Q_criterion (src/lbm/output/Q_criterion.mojo:90-215) and
+ * OutputRequest.get_Q_criterion / Q_criterion_frame (src/lbm/outputrequest.mojo:136-223).
+ * omega comes from finite differences of u, the velocity field lbm_macroscopic wrote for the
+ * same state (u_layout as there), with the reference's rule (finite_difference.mojo:15-136):
+ * a SOLID neighbour sits at half distance, a neighbour outside the grid is skipped and a
+ * derivative with one valid side is 0. S = Pi_neq * (-1 / (2 rho cs^2 tau)) from the
+ * re-streamed, boundary-applied populations (after_odd as for lbm_macroscopic). SOLID nodes
+ * (raw flag) get 0; the reference leaves them unwritten.
+ */
+int lbm_q_criterion(lbm_handle* h, void* q, const void* u, const void* f, const void* bc,
+                    const uint8_t* flags, double tau, int after_odd, int u_layout, void* stream);
+
+/*
  * FP64 totals over valid nodes (not SOLID, not EXCLUDE) of the same moments
  * lbm_macroscopic reports: out_dev[0] = sum rho, out_dev[1+d] = sum rho*u_d.
  * out_dev is device memory of D+1 doubles. Deterministic (fixed reduction

@@ -16,7 +16,8 @@ __all__ = [
     "DOUBLE_BUFFER", "ESOTERIC_PULL", "FLUID_NODE", "SOLID_NODE", "cs", "cs_squared", "LBM_Grid",
     "TiledLayouts", "Lattice", "get_D2Q9", "get_D3Q19", "get_D3Q27", "UnitSystem", "tau_from_re",
     "Solver", "Assembly", "ContextTileTensor", "set_exterior_walls", "initialize_fluid_at_rest",
-    "initialize_equilibrium", "calculate_rho_and_velocity", "reduce_moments",
+    "initialize_equilibrium", "calculate_rho_and_velocity", "calculate_Q_criterion", "reduce_moments",
+    "OutputRequest",
 ]
 
 
@@ -34,7 +35,7 @@ def __getattr__(name):
     if name in ("set_exterior_walls", "initialize_fluid_at_rest", "initialize_equilibrium"):
         from . import preprocess
         return getattr(preprocess, name)
-    if name in ("calculate_rho_and_velocity", "reduce_moments"):
+    if name in ("calculate_rho_and_velocity", "calculate_Q_criterion", "reduce_moments", "OutputRequest"):
         from . import output
         return getattr(output, name)
     if name == "snapshot":

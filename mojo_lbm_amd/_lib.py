@@ -56,6 +56,7 @@ SIGNATURES = {
     "lbm_step_db": ([_vp, _vp, _vp, _vp, _vp, _d, _vp], _i),
     "lbm_step_ep": ([_vp, _vp, _vp, _vp, _d, _i, _vp], _i),
     "lbm_macroscopic": ([_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp], _i),
+    "lbm_q_criterion": ([_vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _i, _vp], _i),
     "lbm_reduce_moments_f64": ([_vp, _vp, _vp, _vp, _vp, _i, _vp], _i),
     "lbm_init_equilibrium": ([_vp, _vp, _d, C.POINTER(_d), _vp], _i),
     "lbm_set_exterior_wall": ([_vp, _vp, _vp, _cp, _i, C.POINTER(_d), _d, _vp], _i),

@@ -239,6 +239,19 @@ int lbm_macroscopic(lbm_handle* hh, void* rho, void* u, const void* f, const voi
     });
 }
 
+int lbm_q_criterion(lbm_handle* hh, void* q, const void* u, const void* f, const void* bc, const uint8_t* flags,
+                    double tau, int after_odd, int u_layout, void* stream) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (!q || !u || !f || !bc || !flags) return set_error(LBM_ERR_INVALID, "NULL buffer");
+    if (u_layout != LBM_U_XYZD && u_layout != LBM_U_DXYZ) return set_error(LBM_ERR_INVALID, "unknown u_layout");
+    if (!(tau > 0.0)) return set_error(LBM_ERR_INVALID, "tau must be positive");
+    const int mode = macro_mode(h, after_odd);
+    return dispatch(h.d, [&](auto l) {
+        return decltype(l)::qcrit(h, q, u, f, bc, flags, tau, mode, u_layout, (hipStream_t)stream);
+    });
+}
+
 int lbm_reduce_moments_f64(lbm_handle* hh, double* out_dev, const void* f, const void* bc, const uint8_t* flags,
                            int after_odd, void* stream) {
     if (int rc = check_handle(hh)) return rc;

@@ -56,8 +56,14 @@ template <class R> struct MacroParams {
     int moving;
     int eqbc;
     uint32_t f_bytes;
+    const R* vel;              // Q-criterion: velocity field in u_layout
+    R* qout;                   // Q-criterion output (row-major x, y, z)
+    R tau;                     // Q-criterion strain-rate tau
     Geo g;
 };
+
+// What macro_kernel writes.
+enum MacroKind { MK_FIELDS = 0, MK_REDUCE = 1, MK_QCRIT = 2 };
 
 // ------------------------------------------------------------ tile region masks
 // Bit (sx+1) + 3 (sy+1) + 9 (sz+1), s in {-1, 0, 1}, of a tile's mask is set
@@ -642,9 +648,46 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
     step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, tile, tile, sfl);
 }
 
-// calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125) and the FP64 moment
-// reduction: re-stream + BC (tau = 0), moments; solid nodes (raw flag == SOLID) report bc.
-template <class L, class S, int MODE, bool DDF, bool REDUCE, bool BIG>
+// get_velocity_gradient (src/lbm/kernels/utils/finite_difference.mojo:15-136), dx = 1: the
+// one-sided differences to the -1 / +1 neighbours along `axis`, a SOLID neighbour (raw flag,
+// from the LDS flag tile) at half distance, blended by left_weight = 1 - left_dx / (left_dx +
+// right_dx), right_weight = 1 - left_weight. Neighbours outside the grid are skipped (no
+// periodic wrap), which leaves a single valid side with weight 0.
+template <class R, int D>
+__device__ __forceinline__ R fd_gradient(const MacroParams<R>& p, const int (&pos)[3], size_t r, const uint8_t* sfl,
+                                         int center, const int (&lstride)[3], int comp, int axis) {
+    const int n[3] = {p.g.nx, p.g.ny, p.g.nz};
+    const size_t rstride[3] = {(size_t)p.g.ny * p.g.nz, (size_t)p.g.nz, 1};
+    const size_t ncell = (size_t)p.g.nx * p.g.ny * p.g.nz;
+    auto U = [&](size_t rr) { return p.u_layout == LBM_U_XYZD ? p.vel[rr * D + comp] : p.vel[comp * ncell + rr]; };
+    const R uc = U(r);
+    R left_grad = 0, left_dx = 0, right_grad = 0, right_dx = 0;
+    const bool left_ok = pos[axis] - 1 >= 0;
+    if (left_ok) {
+        const R ua = U(r - rstride[axis]);
+        const R gdx = sfl[center - lstride[axis]] == LBM_FLAG_SOLID ? (R)0.5 : (R)1;
+        left_grad = (uc - ua) * ((R)1 / gdx);
+        left_dx = gdx;
+    }
+    const bool right_ok = pos[axis] + 1 < n[axis];
+    if (right_ok) {
+        const R ua = U(r + rstride[axis]);
+        const R gdx = sfl[center + lstride[axis]] == LBM_FLAG_SOLID ? (R)0.5 : (R)1;
+        right_grad = (ua - uc) * ((R)1 / gdx);
+        right_dx = gdx;
+    }
+    const R total = left_dx + right_dx;
+    const R left_w = (R)1 - left_dx / total;
+    const R right_w = (R)1 - left_w;
+    return (left_ok || right_ok) ? right_grad * right_w + left_grad * left_w : (R)0;
+}
+
+// calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125), the FP64 moment
+// reduction and calculate_Q_criterion (src/lbm/output/Q_criterion.mojo:90-215): re-stream +
+// BC (the BCs take no tau), then moments, or Q = 0.25 |omega|^2 - 0.5 |S|^2 with omega from
+// finite differences of the velocity field and S from the re-streamed populations.
+// Solid nodes (raw flag == SOLID) report bc in the fields and 0 as Q.
+template <class L, class S, int MODE, bool DDF, int KIND, bool BIG>
 __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroParams<typename S::R> p) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
@@ -669,6 +712,39 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroPar
         }
         apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
     }
+    if (KIND == MK_QCRIT) {
+        const size_t r = ((size_t)t.x * p.g.ny + t.y) * p.g.nz + t.z;
+        R qv = 0;
+        if (t.own != LBM_FLAG_SOLID) {
+            const int pos[3] = {t.x, t.y, t.z};
+            const int ls[3] = {1, t.hx, t.hxy};
+            auto grad = [&](int comp, int axis) { return fd_gradient<R, D>(p, pos, r, sfl, t.center, ls, comp, axis); };
+            R wsq;
+            if (D == 2) {
+                const R w = grad(1, 0) - grad(0, 1);
+                wsq = w * w;
+            } else {
+                const R wx = grad(2, 1) - grad(1, 2);
+                const R wy = grad(0, 2) - grad(2, 0);
+                const R wz = grad(1, 0) - grad(0, 1);
+                wsq = wx * wx;
+                wsq = wsq + wy * wy;
+                wsq = wsq + wz * wz;
+            }
+            const R rho = density<L, R, DDF>(fv);
+            R u[3];
+            velocity<L, R>(fv, rho, u);
+            const R uu = dot_uu<L, R>(u);
+            R fn[Q], pi[6];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fn[q] = fv[q] - feq<L, R, DDF>(q, rho, u, uu);
+            pi_neq<L, R>(fn, pi);
+            qv = (R)0.25 * wsq - (R)0.5 * strain_norm_sq<L, R>(pi, rho, p.tau);
+        }
+        p.qout[r] = qv;
+        return;
+    }
+    constexpr bool REDUCE = KIND == MK_REDUCE;
     R rho = 0, u[3] = {0, 0, 0};
     bool contributes = false;
     if (REDUCE) {

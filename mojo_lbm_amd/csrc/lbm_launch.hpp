@@ -41,6 +41,8 @@ template <class L, class S> struct Launch {
                     int mode, hipStream_t st);
     static int macro(const Handle& h, void* rho, void* u, const void* f, const void* bc, const uint8_t* flags,
                      int mode, int u_layout, hipStream_t st);
+    static int qcrit(const Handle& h, void* q, const void* u, const void* f, const void* bc, const uint8_t* flags,
+                     double tau, int mode, int u_layout, hipStream_t st);
     static int reduce(const Handle& h, double* out, const void* f, const void* bc, const uint8_t* flags, int mode,
                       hipStream_t st);
     static int link_force(const Handle& h, double* out, hipStream_t st);

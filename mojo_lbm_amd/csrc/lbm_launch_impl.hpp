@@ -103,24 +103,24 @@ template <class L, class S> struct LaunchImpl {
         return step_mode<MODE_EP_ODD>(h, p, st);
     }
 
-    template <int MODE, bool DDF, bool REDUCE>
+    template <int MODE, bool DDF, int KIND>
     static int macro_k(const Handle& h, const MacroParams<R>& p, hipStream_t st) {
         if (h.big)
-            hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, REDUCE, true>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
+            hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, KIND, true>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
                                h.lds_bytes, st, p);
         else
-            hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, REDUCE, false>), dim3(h.g.n_tiles),
+            hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, KIND, false>), dim3(h.g.n_tiles),
                                dim3(h.g.tile_cells), h.lds_bytes, st, p);
         return check_launch("macro_kernel");
     }
 
-    template <bool REDUCE>
+    template <int KIND>
     static int macro_any(const Handle& h, const MacroParams<R>& p, int mode, hipStream_t st) {
         const bool ddf = h.d.ddf_shift != 0;
-        if (mode == MODE_DB) return ddf ? macro_k<MODE_DB, true, REDUCE>(h, p, st) : macro_k<MODE_DB, false, REDUCE>(h, p, st);
+        if (mode == MODE_DB) return ddf ? macro_k<MODE_DB, true, KIND>(h, p, st) : macro_k<MODE_DB, false, KIND>(h, p, st);
         if (mode == MODE_EP_EVEN)
-            return ddf ? macro_k<MODE_EP_EVEN, true, REDUCE>(h, p, st) : macro_k<MODE_EP_EVEN, false, REDUCE>(h, p, st);
-        return ddf ? macro_k<MODE_EP_ODD, true, REDUCE>(h, p, st) : macro_k<MODE_EP_ODD, false, REDUCE>(h, p, st);
+            return ddf ? macro_k<MODE_EP_EVEN, true, KIND>(h, p, st) : macro_k<MODE_EP_EVEN, false, KIND>(h, p, st);
+        return ddf ? macro_k<MODE_EP_ODD, true, KIND>(h, p, st) : macro_k<MODE_EP_ODD, false, KIND>(h, p, st);
     }
 
     static MacroParams<R> macro_params(const Handle& h, const void* f, const void* bc, const uint8_t* flags,
@@ -147,14 +147,24 @@ template <class L, class S> struct LaunchImpl {
         p.rho = (R*)rho;
         p.u = (R*)u;
         p.u_layout = u_layout;
-        return macro_any<false>(h, p, mode, st);
+        return macro_any<MK_FIELDS>(h, p, mode, st);
+    }
+
+    static int qcrit(const Handle& h, void* q, const void* u, const void* f, const void* bc, const uint8_t* flags,
+                     double tau, int mode, int u_layout, hipStream_t st) {
+        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
+        p.vel = (const R*)u;
+        p.qout = (R*)q;
+        p.tau = (R)tau;
+        p.u_layout = u_layout;
+        return macro_any<MK_QCRIT>(h, p, mode, st);
     }
 
     static int reduce(const Handle& h, double* out, const void* f, const void* bc, const uint8_t* flags, int mode,
                       hipStream_t st) {
         MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
         p.partial = h.partial;
-        int rc = macro_any<true>(h, p, mode, st);
+        int rc = macro_any<MK_REDUCE>(h, p, mode, st);
         if (rc) return rc;
         if (L::D == 2) hipLaunchKernelGGL((reduce_finish<3>), dim3(1), dim3(256), 0, st, h.partial, h.g.n_tiles, out);
         else hipLaunchKernelGGL((reduce_finish<4>), dim3(1), dim3(256), 0, st, h.partial, h.g.n_tiles, out);
@@ -199,6 +209,11 @@ template <class L, class S>
 int Launch<L, S>::reduce(const Handle& h, double* out, const void* f, const void* bc, const uint8_t* flags, int mode,
                          hipStream_t st) {
     return LaunchImpl<L, S>::reduce(h, out, f, bc, flags, mode, st);
+}
+template <class L, class S>
+int Launch<L, S>::qcrit(const Handle& h, void* q, const void* u, const void* f, const void* bc, const uint8_t* flags,
+                        double tau, int mode, int u_layout, hipStream_t st) {
+    return LaunchImpl<L, S>::qcrit(h, q, u, f, bc, flags, tau, mode, u_layout, st);
 }
 template <class L, class S> int Launch<L, S>::link_force(const Handle& h, double* out, hipStream_t st) {
     return LaunchImpl<L, S>::link_force(h, out, st);

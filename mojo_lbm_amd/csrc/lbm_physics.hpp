@@ -103,7 +103,7 @@ __device__ __forceinline__ void pi_neq(const R (&fn)[L::Q], R (&pi)[6]) {
 // its squared norm (:182-215) and get_Smagorinsky_LES_tau (ops/turbulence.mojo:12-38).
 // The strain rate uses the base tau (steps/collide.mojo:69).
 template <class L, class R>
-__device__ __forceinline__ R smagorinsky_tau(const R (&pi)[6], R rho, R tau, R cs) {
+__device__ __forceinline__ R strain_norm_sq(const R (&pi)[6], R rho, R tau) {
     constexpr int NS = L::D * (L::D + 1) / 2;
     const R cs2 = (R)1 / (R)3;
     const R k = (R)(-1) / ((R)2 * rho * cs2 * tau);
@@ -114,6 +114,12 @@ __device__ __forceinline__ R smagorinsky_tau(const R (&pi)[6], R rho, R tau, R c
         const R ss = s * s;
         snorm = snorm + ((stress_a(L::D, n) != stress_b(L::D, n)) ? ss * (R)2 : ss);
     }
+    return snorm;
+}
+
+template <class L, class R>
+__device__ __forceinline__ R smagorinsky_tau(const R (&pi)[6], R rho, R tau, R cs) {
+    const R snorm = strain_norm_sq<L, R>(pi, rho, tau);
     const R v_eddy = (cs * cs) * sqrt((R)2 * snorm);
     return (R)3 * v_eddy;
 }

@@ -27,6 +27,8 @@
  *   Smagorinsky ........... closed form on a synthetic shear f_neq
  *   KBC ................... gamma = 2 reduces to SRT; conservation; D2Q9 shear table
  *   equilibrium BC ........ closed form (node reset to f_eq of prescribed rho, u)
+ *   Q-criterion ........... solid-body rotation (Q = |Omega|^2), uniform strain (Q = -9 eps^2 / (4 tau^2)),
+ *                           linear shear against a wall at half distance, one-sided edges
  *   LDC D2Q9 Re=100 ....... Ghia et al. centreline fixture (benchmarks/LDC/*), loose physics check
  */
 #include <cmath>
@@ -756,6 +758,113 @@ int reduce_impl(const lbm_desc* d, double* out, const S* f, const T* bc, const u
     return 0;
 }
 
+/* get_velocity_gradient, get_adj_finite_difference and forward_difference,
+ * src/lbm/kernels/utils/finite_difference.mojo:15-136, with dx = 1. A neighbour outside
+ * the grid (no periodic wrap, index.mojo is_index_valid) is skipped; a SOLID neighbour
+ * (raw flag compare) sits at half distance. The two one-sided differences are blended by
+ * left_weight = 1 - left_dx / (left_dx + right_dx), right_weight = 1 - left_weight, so
+ * with a single valid side the valid difference gets weight 0 and the result is 0.
+ * u is the row-major velocity field (x, y, z, d) or (d, x, y, z). */
+template <class T>
+T velocity_gradient(const Geo& g, int D, const T* u, int u_layout, const uint8_t* flags, const int idx[3], int comp,
+                    int axis) {
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const size_t ncell = (size_t)g.nx * g.ny * g.nz;
+    auto U = [&](const int p[3]) {
+        const size_t r = ((size_t)p[0] * g.ny + p[1]) * g.nz + p[2];
+        return u_layout == LBM_U_XYZD ? u[r * D + comp] : u[(size_t)comp * ncell + r];
+    };
+    const T uc = U(idx);
+    T left_grad = 0, left_dx = 0, right_grad = 0, right_dx = 0;
+    int a[3] = {idx[0], idx[1], idx[2]};
+    a[axis] = idx[axis] - 1;
+    const bool left_ok = a[axis] >= 0;
+    if (left_ok) {
+        const T ua = U(a);
+        const T gdx = flags[g.addr(a[0], a[1], a[2], 0, 1)] == LBM_FLAG_SOLID ? (T)0.5 : (T)1;
+        left_grad = (uc - ua) * ((T)1 / gdx);
+        left_dx = gdx;
+    }
+    a[axis] = idx[axis] + 1;
+    const bool right_ok = a[axis] < n[axis];
+    if (right_ok) {
+        const T ua = U(a);
+        const T gdx = flags[g.addr(a[0], a[1], a[2], 0, 1)] == LBM_FLAG_SOLID ? (T)0.5 : (T)1;
+        right_grad = (ua - uc) * ((T)1 / gdx);
+        right_dx = gdx;
+    }
+    const T total = left_dx + right_dx;
+    const T left_w = (T)1 - left_dx / total;
+    const T right_w = (T)1 - left_w;
+    return (left_ok || right_ok) ? right_grad * right_w + left_grad * left_w : (T)0;
+}
+
+/* calculate_Q_criterion and calculate_vorticity, src/lbm/output/Q_criterion.mojo:25-215:
+ * |omega|^2 from finite differences of the velocity input (Vector.norm_squared, a left-to-right
+ * sum from 0), the strain rate S = Pi_neq * (-1 / (2 rho cs^2 tau)) of the re-streamed,
+ * boundary-applied populations (get_strain_rate_tensor, moment.mojo:154-176; its squared
+ * norm with doubled off-diagonals, :182-215), Q = 0.25 |omega|^2 - 0.5 |S|^2. SOLID nodes
+ * (raw flag) get 0; the reference leaves them unwritten. Output row-major (x, y, z). */
+template <class S, class T>
+int q_impl(const lbm_desc* d, T* q_out, const T* u_in, int u_layout, const S* f, const T* bc, const uint8_t* flags,
+           double tau_d, int mode, const uint32_t* link_nodes = nullptr, const float* link_q = nullptr,
+           uint32_t n_links = 0) {
+    Stepper<S, T> st(*d);
+    st.link_nodes = link_nodes;
+    st.link_q = link_q;
+    st.n_links = n_links;
+    const Geo& g = st.g;
+    const long ncell = (long)g.nx * g.ny * g.nz;
+    const int D = st.L.D;
+    const bool ddf = d->ddf_shift != 0;
+    const T tau = (T)tau_d;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (long i = 0; i < ncell; ++i) {
+        int idx[3] = {(int)(i % g.nx), (int)((i / g.nx) % g.ny), (int)(i / ((long)g.nx * g.ny))};
+        const size_t r = ((size_t)idx[0] * g.ny + idx[1]) * g.nz + idx[2];
+        if (flags[g.addr(idx[0], idx[1], idx[2], 0, 1)] == LBM_FLAG_SOLID) {
+            q_out[r] = 0;
+            continue;
+        }
+        auto grad = [&](int comp, int axis) { return velocity_gradient<T>(g, D, u_in, u_layout, flags, idx, comp, axis); };
+        T w[3];
+        int nw;
+        if (D == 2) {
+            w[0] = grad(1, 0) - grad(0, 1);
+            nw = 1;
+        } else {
+            w[0] = grad(2, 1) - grad(1, 2);
+            w[1] = grad(0, 2) - grad(2, 0);
+            w[2] = grad(1, 0) - grad(0, 1);
+            nw = 3;
+        }
+        T wsq = 0;
+        for (int k = 0; k < nw; ++k) wsq += w[k] * w[k];
+        T fv[27];
+        uint8_t pf[27];
+        st.gather(f, bc, flags, idx, mode, fv, pf);
+        const T rho = st.cell.density(fv, ddf);
+        T u[3] = {0, 0, 0};
+        st.cell.velocity(fv, rho, u);
+        const T uu = st.cell.dot_uu(u);
+        T fn[27], pi[6];
+        for (int q = 0; q < st.L.Q; ++q) fn[q] = fv[q] - st.cell.feq(q, rho, u, uu, ddf);
+        st.cell.pi_neq(fn, pi);
+        const T cs2 = (T)1 / (T)3;
+        const T k = (T)(-1) / ((T)2 * rho * cs2 * tau);
+        T snorm = 0;
+        for (int n = 0; n < st.cell.nstress(); ++n) {
+            int a, b;
+            st.cell.stress_pair(n, a, b);
+            const T s = pi[n] * k;
+            const T ss = s * s;
+            snorm += a != b ? ss * (T)2 : ss;
+        }
+        q_out[r] = (T)0.25 * wsq - (T)0.5 * snorm;
+    }
+    return 0;
+}
+
 /* initialize_f_from_func with a uniform velocity and no gradient term,
  * src/lbm/preprocess/initial_condition.mojo:86-178 (esoteric: odd store pattern, :172-174). */
 template <class S, class T>
@@ -903,6 +1012,30 @@ int orc_reduce_moments_links(const lbm_desc* d, double* out, const void* f, cons
     ORC_DISPATCH((reduce_impl<float, float>(d, out, (const float*)f, (const float*)bc, flags, 0, nodes, q, n)),
                  (reduce_impl<uint16_t, float>(d, out, (const uint16_t*)f, (const float*)bc, flags, 0, nodes, q, n)),
                  (reduce_impl<double, double>(d, out, (const double*)f, (const double*)bc, flags, 0, nodes, q, n)))
+}
+
+/* Q-criterion of the current state (after_odd as for orc_macroscopic); u is the velocity field
+ * of orc_macroscopic in u_layout, q the row-major output, both in the grid float type. */
+int orc_q_criterion(const lbm_desc* d, void* q, const void* u, int u_layout, const void* f, const void* bc,
+                    const uint8_t* flags, double tau, int after_odd) {
+    int mode = d->method == LBM_ESOTERIC_PULL ? (after_odd ? 1 : 2) : 0;
+    ORC_DISPATCH((q_impl<float, float>(d, (float*)q, (const float*)u, u_layout, (const float*)f, (const float*)bc, flags, tau, mode)),
+                 (q_impl<uint16_t, float>(d, (float*)q, (const float*)u, u_layout, (const uint16_t*)f, (const float*)bc, flags, tau, mode)),
+                 (q_impl<double, double>(d, (double*)q, (const double*)u, u_layout, (const double*)f, (const double*)bc, flags, tau, mode)))
+}
+
+int orc_q_criterion_links(const lbm_desc* d, void* q, const void* u, int u_layout, const void* f, const void* bc,
+                          const uint8_t* flags, double tau, const uint32_t* nodes, const float* qd, uint32_t n) {
+    ORC_DISPATCH((q_impl<float, float>(d, (float*)q, (const float*)u, u_layout, (const float*)f, (const float*)bc, flags, tau, 0, nodes, qd, n)),
+                 (q_impl<uint16_t, float>(d, (float*)q, (const float*)u, u_layout, (const uint16_t*)f, (const float*)bc, flags, tau, 0, nodes, qd, n)),
+                 (q_impl<double, double>(d, (double*)q, (const double*)u, u_layout, (const double*)f, (const double*)bc, flags, tau, 0, nodes, qd, n)))
+}
+
+/* One finite-difference velocity gradient d u_comp / d x_axis at node (x, y, z), FP64. */
+double orc_velocity_gradient(const lbm_desc* d, const double* u, int u_layout, const uint8_t* flags, int x, int y,
+                             int z, int comp, int axis) {
+    const int idx[3] = {x, y, z};
+    return velocity_gradient<double>(make_geo(d), d->D, u, u_layout, flags, idx, comp, axis);
 }
 
 int orc_init_equilibrium(const lbm_desc* d, void* f, double rho, const double* u) {

@@ -57,6 +57,12 @@ def lib():
         for n in ("orc_step_db", "orc_step_ep", "orc_macroscopic", "orc_reduce_moments",
                   "orc_init_equilibrium", "orc_set_exterior_wall"):
             getattr(L, n).restype = i
+        L.orc_q_criterion.argtypes = [pd, vp, vp, i, vp, vp, vp, d, i]
+        L.orc_q_criterion.restype = i
+        L.orc_q_criterion_links.argtypes = [pd, vp, vp, i, vp, vp, vp, d, vp, vp, C.c_uint32]
+        L.orc_q_criterion_links.restype = i
+        L.orc_velocity_gradient.argtypes = [pd, vp, i, vp, i, i, i, i, i]
+        L.orc_velocity_gradient.restype = d
         L.orc_set_threads(L.orc_max_threads())
         _lib = L
     return _lib
@@ -175,6 +181,26 @@ def macroscopic(desc, f, bc, flags, after_odd=False, u_layout=0, links=None):
     _ok(lib().orc_macroscopic(C.byref(desc), _p(rho), _p(u), _p(f), _p(bc), _p(flags), int(after_odd),
                               int(u_layout)), "macroscopic")
     return rho, u
+
+
+def q_criterion(desc, u, f, bc, flags, tau, after_odd=False, u_layout=0, links=None):
+    """Q-criterion (row-major x, y, z) of the state; u is the velocity field in u_layout."""
+    q = np.zeros(n_cells(desc), r_dtype(desc))
+    u = np.ascontiguousarray(u, r_dtype(desc))
+    if links is not None:
+        nodes, qd = _links(links)
+        _ok(lib().orc_q_criterion_links(C.byref(desc), _p(q), _p(u), int(u_layout), _p(f), _p(bc), _p(flags),
+                                        float(tau), _p(nodes), _p(qd), len(nodes)), "q_criterion_links")
+        return q
+    _ok(lib().orc_q_criterion(C.byref(desc), _p(q), _p(u), int(u_layout), _p(f), _p(bc), _p(flags), float(tau),
+                              int(after_odd)), "q_criterion")
+    return q
+
+
+def velocity_gradient(desc, u, flags, x, y, z, comp, axis, u_layout=0) -> float:
+    """One finite-difference gradient d u_comp / d x_axis (FP64 velocity field)."""
+    u = np.ascontiguousarray(u, np.float64)
+    return lib().orc_velocity_gradient(C.byref(desc), _p(u), int(u_layout), _p(flags), x, y, z, comp, axis)
 
 
 def reduce_moments(desc, f, bc, flags, after_odd=False, links=None):

@@ -272,6 +272,19 @@ int lbm_les_samples(const float* coarse, int32_t cx, int32_t cy, int32_t cz, dou
 int lbm_init_fields(lbm_handle* h, void* f, const void* rho, const void* u, void* stream);
 
 /*
+ * f from a velocity field on the device: replaces initialize_f_from_func
+ * (src/lbm/preprocess/initial_condition.mojo:86-178) with its non-equilibrium term fi_neq
+ * (:181-255). Per node f_q = f_eq(rho, u) + fi_neq, fi_neq = -w_q rho (tau - 1) / cs^2 *
+ * sum_ab (c_a c_b - cs^2 delta_ab) S_ab with S_ab = (g_ab + g_ba) / 2, in the reference's
+ * operation order; stored in the double-buffer layout or, for esoteric pull, the odd store
+ * pattern the reference uses (:172-174). u (row-major (x,y,z,d)) and grad (row-major
+ * (x,y,z,a,b) = d u_a / d x_b, or NULL for no fi_neq) are lattice-unit values in the grid
+ * float type covering the x-slab [x0, x0 + nx_slab), so a large grid is filled slab by slab.
+ */
+int lbm_init_f_from_fields(lbm_handle* h, void* f, double rho, const void* u, const void* grad, double tau,
+                           int32_t x0, int32_t nx_slab, void* stream);
+
+/*
  * Launch tuning of one handle (no reference counterpart: the reference fixes
  * the launch shape at compile time, src/lbm/grid.mojo:99-100).
  *   LBM_OPT_XCD_REMAP: 1 = give each of the 8 XCDs a contiguous run of tiles

@@ -15,9 +15,9 @@ __all__ = [
     "LBM_Config", "DoubleBufferConfig", "EsotericPullConfig", "Flags", "LBM_method", "Collisions",
     "DOUBLE_BUFFER", "ESOTERIC_PULL", "FLUID_NODE", "SOLID_NODE", "cs", "cs_squared", "LBM_Grid",
     "TiledLayouts", "Lattice", "get_D2Q9", "get_D3Q19", "get_D3Q27", "UnitSystem", "tau_from_re",
-    "Solver", "Assembly", "ContextTileTensor", "set_exterior_walls", "initialize_fluid_at_rest",
-    "initialize_equilibrium", "calculate_rho_and_velocity", "calculate_Q_criterion", "reduce_moments",
-    "OutputRequest",
+    "Solver", "Assembly", "ContextTileTensor", "set_exterior_walls", "set_exterior_walls_with_func",
+    "initialize_fluid_at_rest", "initialize_equilibrium", "initialize_f_from_func", "calculate_rho_and_velocity",
+    "calculate_Q_criterion", "reduce_moments", "OutputRequest",
 ]
 
 
@@ -32,7 +32,8 @@ def __getattr__(name):
     if name == "ContextTileTensor":
         from .tensor import ContextTileTensor
         return ContextTileTensor
-    if name in ("set_exterior_walls", "initialize_fluid_at_rest", "initialize_equilibrium"):
+    if name in ("set_exterior_walls", "set_exterior_walls_with_func", "initialize_fluid_at_rest",
+                "initialize_equilibrium", "initialize_f_from_func"):
         from . import preprocess
         return getattr(preprocess, name)
     if name in ("calculate_rho_and_velocity", "calculate_Q_criterion", "reduce_moments", "OutputRequest"):

@@ -71,6 +71,7 @@ SIGNATURES = {
                         C.c_int32, C.c_int32, _vp, _vp], _i),
     "lbm_les_samples": ([_vp, C.c_int32, C.c_int32, C.c_int32, _d, _d, _vp, C.c_int64, _vp, _vp, _vp], _i),
     "lbm_init_fields": ([_vp, _vp, _vp, _vp, _vp], _i),
+    "lbm_init_f_from_fields": ([_vp, _vp, _d, _vp, _vp, _d, C.c_int32, C.c_int32, _vp], _i),
     "lbm_link_force": ([_vp, _vp, _vp], _i),
     "lbm_f16c_convert": ([_vp, _vp, _sz, _i, _vp], _i),
     "lbm_set_option": ([_vp, _i, _i], _i),

@@ -1,8 +1,8 @@
 // lbm_les_data.hip -- training-data kernels of the learned LES closure (include/lbm_hip.h,
 // "Learned closure"): box filter of fine fields onto a coarse grid, a-priori features and
 // eddy-viscosity targets on the coarse grid, and per-node equilibrium initialisation from
-// fields. No reference counterpart; the initialisation is initialize_f_from_func without the
-// gradient term (src/lbm/preprocess/initial_condition.mojo:86-178) for a given field.
+// fields. No reference counterpart for the first two; the initialisations are
+// initialize_f_from_func (src/lbm/preprocess/initial_condition.mojo:86-234) for given fields.
 #include <cmath>
 #include <string>
 
@@ -155,6 +155,109 @@ __global__ void __launch_bounds__(256) init_fields_kernel(typename S::T* __restr
     for (int q = 0; q < L::Q; ++q) S::st(f + (tl * L::Q + q) * (size_t)g.tile_cells + loc, feq<L, R, DDF>(q, rr, uu3, uu));
 }
 
+// Element index in f of node (x, y, z), population q.
+__device__ __forceinline__ size_t f_elem(const Geo& g, int x, int y, int z, int Q, int q) {
+    const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
+    const size_t loc = (size_t)((x & (tx - 1)) + ((y & (ty - 1)) << g.txb) + ((z & (tz - 1)) << (g.txb + g.tyb)));
+    const size_t tl = (size_t)(x >> g.txb) + (size_t)g.Tx * ((size_t)(y >> g.tyb) + (size_t)g.Ty * (size_t)(z >> g.tzb));
+    return (tl * Q + q) * (size_t)g.tile_cells + loc;
+}
+
+// initialize_f_from_func (src/lbm/preprocess/initial_condition.mojo:86-178) over given fields:
+// f_q = f_eq(rho, u) plus, when grad is given, fi_neq (:181-234):
+//   fi_neq = sum_ab (c_a c_b - cs^2 delta_ab) 0.5 (g_ab + g_ba), then
+//   fi_neq *= ((-w_q rho) tau / cs^2) ((tau - 1) / tau),
+// with g_ab = d u_a / d x_b. Stored in the double-buffer layout or, for esoteric pull, with the
+// odd store pattern (:172-174): f_p (p odd) to slot p of x + c_p, the rest to x. u row-major
+// (x, y, z, d) and grad row-major (x, y, z, a, b) cover the x-slab [x0, x0 + nxc).
+template <class L, class S, bool DDF, bool EP, bool NEQ>
+__global__ void __launch_bounds__(256) init_func_kernel(typename S::T* __restrict__ f, typename S::R rho,
+                                                        const typename S::R* __restrict__ u,
+                                                        const typename S::R* __restrict__ grad, typename S::R tau,
+                                                        int x0, int nxc, Geo g) {
+    using R = typename S::R;
+    constexpr int D = L::D, Q = L::Q;
+    const int64_t nyz = (int64_t)g.ny * g.nz;
+    const int64_t rl = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (rl >= (int64_t)nxc * nyz) return;
+    const int x = x0 + (int)(rl / nyz), y = (int)((rl / g.nz) % g.ny), z = (int)(rl % g.nz);
+    R uv[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) uv[d] = u[rl * D + d];
+    const R uu = dot_uu<L, R>(uv);
+    R fv[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fv[q] = feq<L, R, DDF>(q, rho, uv, uu);
+    if (NEQ) {
+        R gr[3][3];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) gr[a][b] = grad[(rl * D + a) * D + b];
+        const R cs2 = (R)1 / (R)3;
+        const R tail = (tau - (R)1) / tau;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            R acc = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    const R sab = (R)0.5 * (gr[a][b] + gr[b][a]);
+                    R qiab = (R)L::c(q, a) * (R)L::c(q, b);
+                    if (a == b) qiab = qiab - cs2;
+                    acc = acc + qiab * sab;
+                }
+            const R pre = (((-weight<L, R>(q)) * rho) * tau) * (R)3;
+            acc = acc * (pre * tail);
+            fv[q] = fv[q] + acc;
+        }
+    }
+    if (!EP) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) S::st(f + f_elem(g, x, y, z, Q, q), fv[q]);
+        return;
+    }
+    S::st(f + f_elem(g, x, y, z, Q, 0), fv[0]);
+#pragma unroll
+    for (int p = 1; p < Q; p += 2) {
+        int xn = x + L::c(p, 0), yn = y + L::c(p, 1), zn = z + L::c(p, 2);
+        xn += xn < 0 ? g.nx : 0;
+        xn -= xn >= g.nx ? g.nx : 0;
+        yn += yn < 0 ? g.ny : 0;
+        yn -= yn >= g.ny ? g.ny : 0;
+        zn += zn < 0 ? g.nz : 0;
+        zn -= zn >= g.nz ? g.nz : 0;
+        S::st(f + f_elem(g, xn, yn, zn, Q, p), fv[p]);
+        S::st(f + f_elem(g, x, y, z, Q, p + 1), fv[p + 1]);
+    }
+}
+
+template <class L, class S, bool DDF, bool EP>
+static void init_func_launch(const Handle& h, void* f, double rho, const void* u, const void* grad, double tau,
+                             int x0, int nxc, unsigned blocks, hipStream_t st) {
+    using R = typename S::R;
+    if (grad)
+        hipLaunchKernelGGL((init_func_kernel<L, S, DDF, EP, true>), dim3(blocks), dim3(256), 0, st,
+                           (typename S::T*)f, (R)rho, (const R*)u, (const R*)grad, (R)tau, x0, nxc, h.g);
+    else
+        hipLaunchKernelGGL((init_func_kernel<L, S, DDF, EP, false>), dim3(blocks), dim3(256), 0, st,
+                           (typename S::T*)f, (R)rho, (const R*)u, (const R*)nullptr, (R)tau, x0, nxc, h.g);
+}
+
+template <class L, class S>
+int init_func(const Handle& h, void* f, double rho, const void* u, const void* grad, double tau, int x0, int nxc,
+              hipStream_t st) {
+    const int64_t n = (int64_t)nxc * h.g.ny * h.g.nz;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    const bool ddf = h.d.ddf_shift != 0, ep = h.d.method == LBM_ESOTERIC_PULL;
+    if (ddf && ep) init_func_launch<L, S, true, true>(h, f, rho, u, grad, tau, x0, nxc, blocks, st);
+    else if (ddf) init_func_launch<L, S, true, false>(h, f, rho, u, grad, tau, x0, nxc, blocks, st);
+    else if (ep) init_func_launch<L, S, false, true>(h, f, rho, u, grad, tau, x0, nxc, blocks, st);
+    else init_func_launch<L, S, false, false>(h, f, rho, u, grad, tau, x0, nxc, blocks, st);
+    return check_launch("init_func_kernel");
+}
+
 template <class L, class S>
 int init_fields(const Handle& h, void* f, const void* rho, const void* u, hipStream_t st) {
     using R = typename S::R;
@@ -198,6 +301,29 @@ int lbm_init_fields(lbm_handle* hh, void* f, const void* rho, const void* u, voi
     const hipStream_t st = (hipStream_t)stream;
 #define LBM_CASE(QQ, SS, SID) \
     if (h.d.Q == QQ && h.d.storage == SID) return init_fields<Lattice<QQ>, SS>(h, f, rho, u, st);
+    LBM_CASE(9, StoreF32, LBM_STORE_F32)
+    LBM_CASE(9, StoreF16C, LBM_STORE_F16C)
+    LBM_CASE(9, StoreF64, LBM_STORE_F64)
+    LBM_CASE(19, StoreF32, LBM_STORE_F32)
+    LBM_CASE(19, StoreF16C, LBM_STORE_F16C)
+    LBM_CASE(19, StoreF64, LBM_STORE_F64)
+    LBM_CASE(27, StoreF32, LBM_STORE_F32)
+    LBM_CASE(27, StoreF16C, LBM_STORE_F16C)
+    LBM_CASE(27, StoreF64, LBM_STORE_F64)
+#undef LBM_CASE
+    return set_error(LBM_ERR_INVALID, "no kernel set for this lattice / storage");
+}
+
+int lbm_init_f_from_fields(lbm_handle* hh, void* f, double rho, const void* u, const void* grad, double tau,
+                           int32_t x0, int32_t nx_slab, void* stream) {
+    if (!hh || !f || !u) return set_error(LBM_ERR_INVALID, "NULL argument");
+    const Handle& h = *reinterpret_cast<Handle*>(hh);
+    if (x0 < 0 || nx_slab <= 0 || (int64_t)x0 + nx_slab > h.g.nx)
+        return set_error(LBM_ERR_INVALID, "the x-slab [x0, x0 + nx_slab) must lie inside the grid");
+    if (grad && !(tau != 0.0)) return set_error(LBM_ERR_INVALID, "the non-equilibrium term needs tau != 0");
+    const hipStream_t st = (hipStream_t)stream;
+#define LBM_CASE(QQ, SS, SID) \
+    if (h.d.Q == QQ && h.d.storage == SID) return init_func<Lattice<QQ>, SS>(h, f, rho, u, grad, tau, x0, nx_slab, st);
     LBM_CASE(9, StoreF32, LBM_STORE_F32)
     LBM_CASE(9, StoreF16C, LBM_STORE_F16C)
     LBM_CASE(9, StoreF64, LBM_STORE_F64)

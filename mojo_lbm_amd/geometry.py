@@ -1,7 +1,9 @@
-"""Immersed circles / spheres with interpolated bounce-back links (host side).
+"""Immersed circles / spheres / boxes and interpolated bounce-back links (host side).
 
-add_sphere marks nodes with |x - c|^2 <= r^2 SOLID (src/lbm/geometry/primatives.mojo:110-161,
-in lattice units). sphere_links marks the fluid nodes that pull from such a node
+add_sphere / add_circle / add_box mark SOLID the nodes whose physical coordinates
+(index * dx + origin, in the grid float type) lie in the sphere or box, scanning the
+same bounding box as the reference (src/lbm/geometry/primatives.mojo:110-266), with
+numpy instead of a per-node loop. sphere_links marks the fluid nodes that pull from such a node
 FLUID_SOLID_BOUNDARY and returns their link records for lbm_set_links: the
 fraction of each fluid -> solid link lying in the fluid, the root in [0, 1] of
 |x_f + t (x_s - x_f) - c|^2 = r^2 (get_q_for_sphere, src/lbm/geometry/rigidSphere.mojo:132-170).
@@ -61,12 +63,77 @@ def link_fractions(directions: np.ndarray, inside: np.ndarray, center, radius: f
     return qd
 
 
-def add_sphere(grid: LBM_Grid, flags_host: np.ndarray, center, radius: float) -> np.ndarray:
-    """Marks nodes inside the sphere (circle for D = 2) SOLID in a tiled flag array; returns the
-    boolean node mask [nx, ny, nz]."""
-    inside = sphere_mask(grid.shape, center, radius, grid.D)
-    xs, ys, zs = np.nonzero(inside)
-    flags_host[grid.layouts.index(xs, ys, zs, 0, 1)] = Flags.SOLID
+def _flags_host(flags):
+    return flags.cpu() if hasattr(flags, "cpu") and not isinstance(flags, np.ndarray) else flags
+
+
+def _bbox(grid: LBM_Grid, center, half):
+    """Index ranges of the bounding box, primatives.mojo:138-146 (floor division, clipped to the grid)."""
+    out = []
+    for i in range(3):
+        if i < grid.D:
+            lo, hi = center[i] - half[i], center[i] + half[i]
+            n_min = max(0, int((lo - grid.origin[i]) // grid.dx))
+            n_max = min(grid.shape[i], int((hi - grid.origin[i]) // grid.dx + 1))
+            out.append(np.arange(n_min, max(n_min, n_max)))
+        else:
+            out.append(np.arange(0, 1))
+    return out
+
+
+def _box_coords(grid: LBM_Grid, rng):
+    xs, ys, zs = np.meshgrid(*rng, indexing="ij")
+    dt = grid.float_dtype
+    dx = dt(grid.dx)
+    pts = [dt(1) * np.asarray(i, dt) * dx + dt(o) for i, o in zip((xs, ys, zs), grid.origin)]
+    return xs, ys, zs, pts
+
+
+def add_sphere(grid: LBM_Grid, flags, center, radius: float) -> np.ndarray:
+    """primatives.mojo:110-161: SOLID where sum_i (x_i - c_i)^2 <= r^2 (all three axes, in the grid
+    float type). flags is a ContextTileTensor or a tiled host array; returns the boolean node
+    mask [nx, ny, nz] of the nodes it marked."""
+    if len(center) != 3:
+        raise ValueError(f"centre must be a list of 3 floats got a len of {len(center)} instead")
+    dt = grid.float_dtype
+    c = [dt(v) for v in center]
+    r = dt(radius)
+    xs, ys, zs, pts = _box_coords(grid, _bbox(grid, [float(v) for v in c], [float(r)] * 3))
+    d2 = (pts[0] - c[0]) ** 2
+    d2 = d2 + (pts[1] - c[1]) ** 2
+    d2 = d2 + (pts[2] - c[2]) ** 2
+    sel = d2 <= r * r
+    fl = _flags_host(flags)
+    fl[grid.layouts.index(xs[sel], ys[sel], zs[sel], 0, 1)] = Flags.SOLID
+    inside = np.zeros(grid.shape, bool)
+    inside[xs[sel], ys[sel], zs[sel]] = True
+    return inside
+
+
+def add_circle(grid: LBM_Grid, flags, center, radius: float) -> np.ndarray:
+    """primatives.mojo:163-183: add_sphere on a 2-D grid."""
+    if grid.D != 2:
+        raise ValueError("Circle only valid for 2D")
+    return add_sphere(grid, flags, center, radius)
+
+
+def add_box(grid: LBM_Grid, flags, center, box_radius) -> np.ndarray:
+    """primatives.mojo:186-243 / check_box_axis :246-266: SOLID where c - h <= x <= c + h on every axis."""
+    if len(center) != 3:
+        raise ValueError(f"centre must be a list of 3 floats got a len of {len(center)} instead")
+    if len(box_radius) != 3:
+        raise ValueError(f"lengths must be a list of 3 floats got a len of {len(box_radius)} instead")
+    dt = grid.float_dtype
+    c = [dt(v) for v in center]
+    h = [dt(v) for v in box_radius]
+    xs, ys, zs, pts = _box_coords(grid, _bbox(grid, [float(v) for v in c], [float(v) for v in h]))
+    sel = np.ones(xs.shape, bool)
+    for i in range(3):
+        sel &= ((c[i] - h[i]) <= pts[i]) & (pts[i] <= (c[i] + h[i]))
+    fl = _flags_host(flags)
+    fl[grid.layouts.index(xs[sel], ys[sel], zs[sel], 0, 1)] = Flags.SOLID
+    inside = np.zeros(grid.shape, bool)
+    inside[xs[sel], ys[sel], zs[sel]] = True
     return inside
 
 

@@ -27,6 +27,8 @@
  *   Smagorinsky ........... closed form on a synthetic shear f_neq
  *   KBC ................... gamma = 2 reduces to SRT; conservation; D2Q9 shear table
  *   equilibrium BC ........ closed form (node reset to f_eq of prescribed rho, u)
+ *   initialize_f_from_func  moments of f_eq(rho, u(x)); Pi_neq = -2 rho cs^2 (tau - 1) S of the gradient
+ *                           term; esoteric store = the double-buffer fill moved by the odd pattern
  *   Q-criterion ........... solid-body rotation (Q = |Omega|^2), uniform strain (Q = -9 eps^2 / (4 tau^2)),
  *                           linear shear against a wall at half distance, one-sided edges
  *   LDC D2Q9 Re=100 ....... Ghia et al. centreline fixture (benchmarks/LDC/*), loose physics check
@@ -887,6 +889,53 @@ int init_impl(const lbm_desc* d, S* f, double rho_d, const double* u_d) {
     return 0;
 }
 
+/* initialize_f_from_func, src/lbm/preprocess/initial_condition.mojo:86-178, for given fields, with
+ * fi_neq (:181-234) and calculate_Sab (:236-255) when grad is given:
+ *   fi_neq = 0; for a, b: fi_neq += (c_a c_b - cs^2 [a == b]) * 0.5 (g_ab + g_ba);
+ *   fi_neq *= ((-w rho) tau / cs^2) * ((tau - 1) / tau); f_q = f_eq + fi_neq,
+ * stored in the double-buffer layout or with the esoteric odd store pattern (:172-174).
+ * u: row-major (x, y, z, d); grad: row-major (x, y, z, a, b) = d u_a / d x_b, or NULL. */
+template <class S, class T>
+int init_func_impl(const lbm_desc* d, S* f, double rho_d, const T* u, const T* grad, double tau_d) {
+    Stepper<S, T> st(*d);
+    const Geo& g = st.g;
+    const int D = st.L.D, Q = st.L.Q;
+    const T rho = (T)rho_d, tau = (T)tau_d;
+    const T cs2 = (T)1 / (T)3;
+    const bool ddf = d->ddf_shift != 0;
+    for (int x = 0; x < g.nx; ++x)
+        for (int y = 0; y < g.ny; ++y)
+            for (int z = 0; z < g.nz; ++z) {
+                const size_t r = ((size_t)x * g.ny + y) * g.nz + z;
+                T uv[3] = {0, 0, 0};
+                for (int dd = 0; dd < D; ++dd) uv[dd] = u[r * D + dd];
+                const T uu = st.cell.dot_uu(uv);
+                T fv[27];
+                for (int q = 0; q < Q; ++q) {
+                    T fi = st.cell.feq(q, rho, uv, uu, ddf);
+                    if (grad) {
+                        const T* gr = grad + r * D * D;
+                        T neq = 0;
+                        for (int a = 0; a < D; ++a)
+                            for (int b = 0; b < D; ++b) {
+                                const T sab = (T)0.5 * (gr[a * D + b] + gr[b * D + a]);
+                                T qiab = (T)st.L.c[q][a] * (T)st.L.c[q][b];
+                                if (a == b) qiab -= cs2;
+                                neq += qiab * sab;
+                            }
+                        neq *= (((-st.cell.w[q]) * rho) * tau * (T)3) * ((tau - (T)1) / tau);
+                        fi += neq;
+                    }
+                    fv[q] = fi;
+                }
+                const int idx[3] = {x, y, z};
+                if (d->method == LBM_ESOTERIC_PULL) st.store_ep(f, idx, false, fv);
+                else
+                    for (int q = 0; q < Q; ++q) Store<S, T>::store(f, g.addr(x, y, z, q, st.K()), fv[q]);
+            }
+    return 0;
+}
+
 int check(const lbm_desc* d) {
     if (!d) return -1;
     bool lat_ok = (d->D == 2 && d->Q == 9) || (d->D == 3 && (d->Q == 19 || d->Q == 27));
@@ -1036,6 +1085,13 @@ double orc_velocity_gradient(const lbm_desc* d, const double* u, int u_layout, c
                              int z, int comp, int axis) {
     const int idx[3] = {x, y, z};
     return velocity_gradient<double>(make_geo(d), d->D, u, u_layout, flags, idx, comp, axis);
+}
+
+/* f from a row-major velocity field (and gradient field, or NULL) in the grid float type. */
+int orc_init_f_from_fields(const lbm_desc* d, void* f, double rho, const void* u, const void* grad, double tau) {
+    ORC_DISPATCH((init_func_impl<float, float>(d, (float*)f, rho, (const float*)u, (const float*)grad, tau)),
+                 (init_func_impl<uint16_t, float>(d, (uint16_t*)f, rho, (const float*)u, (const float*)grad, tau)),
+                 (init_func_impl<double, double>(d, (double*)f, rho, (const double*)u, (const double*)grad, tau)))
 }
 
 int orc_init_equilibrium(const lbm_desc* d, void* f, double rho, const double* u) {

@@ -63,6 +63,8 @@ def lib():
         L.orc_q_criterion_links.restype = i
         L.orc_velocity_gradient.argtypes = [pd, vp, i, vp, i, i, i, i, i]
         L.orc_velocity_gradient.restype = d
+        L.orc_init_f_from_fields.argtypes = [pd, vp, d, vp, vp, d]
+        L.orc_init_f_from_fields.restype = i
         L.orc_set_threads(L.orc_max_threads())
         _lib = L
     return _lib
@@ -217,6 +219,14 @@ def reduce_moments(desc, f, bc, flags, after_odd=False, links=None):
 def init_equilibrium(desc, f, rho=1.0, u=None):
     ua = None if u is None else np.ascontiguousarray(u, np.float64)
     _ok(lib().orc_init_equilibrium(C.byref(desc), _p(f), float(rho), None if ua is None else _p(ua)), "init")
+
+
+def init_f_from_fields(desc, f, rho, u, grad=None, tau=1.0):
+    """initialize_f_from_func over a row-major velocity field (and gradient field) in the grid float type."""
+    u = np.ascontiguousarray(u, r_dtype(desc))
+    g = None if grad is None else np.ascontiguousarray(grad, r_dtype(desc))
+    _ok(lib().orc_init_f_from_fields(C.byref(desc), _p(f), float(rho), _p(u), None if g is None else _p(g),
+                                     float(tau)), "init_f_from_fields")
 
 
 def set_exterior_wall(desc, flags, bc, side: str, flag: int, u=None, rho=float("nan")):

@@ -206,7 +206,7 @@ def obstacle_case(lat="D2Q9", storage=0, ddf=0, seed=0, amp=1e-3, u_in=0.05, col
     from mojo_lbm_amd import geometry as G
 
     if lat == "D2Q9":
-        n, center, radius = (64, 32, 1), (16.3, 15.6), 5.3
+        n, center, radius = (64, 32, 1), (16.3, 15.6, 0.0), 5.3
     else:
         n, center, radius = (32, 16, 16), (10.4, 7.7, 8.1), 4.2
     desc = make_desc(lat, n, storage=storage, ddf=ddf, eq=True, collision=collision, les=les)

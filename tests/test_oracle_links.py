@@ -112,7 +112,7 @@ def test_bouzidi_conserves_mass_in_a_closed_box(lat):
     grid = U.grid_of(desc)
     from mojo_lbm_amd import geometry as G
 
-    c = (15.3, 16.6) if lat == "D2Q9" else (7.4, 8.3, 7.9)
+    c = (15.3, 16.6, 0.0) if lat == "D2Q9" else (7.4, 8.3, 7.9)
     r = 5.2 if lat == "D2Q9" else 3.6
     inside = G.add_sphere(grid, flags, c, r)
     nodes, qdist = G.sphere_links(grid, flags, c, r, inside)

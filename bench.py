@@ -410,7 +410,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default=None,
-                    help="run one variant: fp32 | f16c | f16c_ep | fp32_ep | fp32_periodic | f16c_periodic | cylinders | les")
+                    help="run one variant: fp32 | f16c | f16c_ep | fp32_ep | fp32_periodic | f16c_periodic | cylinders | les "
+                         "| outputs")
     ap.add_argument("--xcd-remap", type=int, default=None, help="override LBM_OPT_XCD_REMAP (0 or 1)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain 1 GiB -> 1 GiB device copy kernel (for scaling rocprofv3 byte counters)")
@@ -425,6 +426,14 @@ def main():
     args = ap.parse_args()
     if args.only == "les":
         print(json.dumps(les_step_times()), flush=True)
+        return
+    if args.only == "outputs":
+        N, u_lid = args.n, 0.1
+        from mojo_lbm_amd import _lib
+
+        d = _lib.LbmDesc(D=3, Q=19, nx=N, ny=N, nz=N, tx=8, ty=8, tz=8, method=0, collision=0, storage=0,
+                         ddf_shift=0, les=0, include_moving_boundary=1, included_bcs=3, smagorinsky_cs=0.1)
+        print(json.dumps(output_times(d, 0.5 + 3.0 * u_lid * N / 100.0, u_lid)), flush=True)
         return
     if args.cylinder_compare or args.only == "cylinders":
         tile = tuple(int(v) for v in args.cylinder_tile.split("x")) if args.cylinder_tile else None

@@ -78,31 +78,34 @@ template <class L, class R> __host__ __device__ __forceinline__ constexpr R weig
 // equal to the value times 2^-112 (e' > 0: exponent e' - 127 instead of
 // e' - 15; e' = 0: the fp32 denormal m * 2^-137 instead of m * 2^-25), so one
 // exact multiply by 2^112 restores it. Needs fp32 denormal inputs, which the
-// library keeps (IEEE mode, no flush).
+// library keeps (IEEE mode, no flush). The code is sign-extended first (a signed
+// 16-bit load): shifted left by 12 its sign lands on bits 27..31, and one mask
+// keeps bit 31 and the magnitude.
 __device__ __forceinline__ float f16c_decode(uint32_t h) {
-    const uint32_t bits = ((h & 0x7FFFu) << 12) | ((h & 0x8000u) << 16);
-    return __uint_as_float(bits) * 0x1p112f;
+    const uint32_t hs = (uint32_t)(int32_t)(int16_t)(uint16_t)h;
+    return __uint_as_float((hs << 12) & 0x87FFFFFFu) * 0x1p112f;
 }
 
-// Encode: in the normal range adding half a code ulp (2^11 in fp32 bits)
-// and truncating 12 bits rounds half away from zero, and subtracting 112 << 11
-// rebiases the exponent; below it the magnitude times 2^25 (exact) counts
-// subnormal code steps, t = r + frac with r = floor(t) and frac = t - r both
-// exact, and frac >= 1/2 rounds half up. (floor(t + 0.5) would not do: the
-// sum itself can round up to the next integer.) Magnitudes of 2 - 2^-12 and
-// above, infinities and NaN saturate to 0x7FFF.
+// Encode, for |x| = a (fp32 bits without the sign):
+//   normal range: adding half a code ulp (2^11 in fp32 bits) and truncating 12
+//   bits rounds half away from zero, and subtracting 112 << 11 rebiases the
+//   exponent: (a + 2^11 - (112 << 23)) >> 12, taken signed, is negative below
+//   2^-15 and at most the subnormal code up to 2^-14;
+//   subnormal range: with |x| clamped to 2^-14, t = |x| 2^26 is exact and below
+//   4097, floor(t) is exact, and (floor(t) + 1) >> 1 rounds |x| 2^25 half up
+//   (at most 2048, the first normal code).
+// The code is the larger of the two, clamped to 0x7FFF, which also saturates
+// magnitudes of 2 - 2^-12 and above, infinities and NaN. Checked against the
+// format definition on all 2^32 inputs.
 __device__ __forceinline__ uint32_t f16c_encode(float x) {
     const uint32_t bits = __float_as_uint(x);
-    const uint32_t s = (bits >> 16) & 0x8000u;
     const uint32_t a = bits & 0x7FFFFFFFu;
-    const uint32_t rn = a + 0x800u;
-    const uint32_t normal = (rn >> 12) - (112u << 11);
-    const float t = fminf(__uint_as_float(a) * 0x1p25f, 4096.0f);
-    const float r = floorf(t);
-    const uint32_t sub = (uint32_t)r + (t - r >= 0.5f ? 1u : 0u);
-    uint32_t code = rn >= (113u << 23) ? normal : sub;
-    code = a >= 0x3FFFF800u ? 0x7FFFu : code;
-    return s | code;
+    const int32_t normal = (int32_t)(a - 0x37FFF800u) >> 12;
+    const float t = __uint_as_float(min(a, 0x38800000u)) * 0x1p26f;
+    const int32_t sub = (int32_t)(((uint32_t)t + 1u) >> 1);
+    int32_t code;  // min(max(normal, sub), 0x7FFF) in one instruction (sub <= 0x7FFF)
+    __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(code) : "v"(normal), "v"(sub), "s"(0x7FFF));
+    return ((bits >> 16) & 0x8000u) | (uint32_t)code;
 }
 
 // ----------------------------------------------------------------- storage

@@ -59,6 +59,7 @@ template <class R> struct MacroParams {
     const R* vel;              // Q-criterion: velocity field in u_layout
     R* qout;                   // Q-criterion output (row-major x, y, z)
     R tau;                     // Q-criterion strain-rate tau
+    uint32_t su_offset;        // Q-criterion: byte offset of the LDS velocity tile
     Geo g;
 };
 
@@ -652,27 +653,26 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
 // one-sided differences to the -1 / +1 neighbours along `axis`, a SOLID neighbour (raw flag,
 // from the LDS flag tile) at half distance, blended by left_weight = 1 - left_dx / (left_dx +
 // right_dx), right_weight = 1 - left_weight. Neighbours outside the grid are skipped (no
-// periodic wrap), which leaves a single valid side with weight 0.
+// periodic wrap), which leaves a single valid side with weight 0. Velocities come from the
+// LDS velocity tile su (D values per halo cell, the flag tile's cell order).
 template <class R, int D>
-__device__ __forceinline__ R fd_gradient(const MacroParams<R>& p, const int (&pos)[3], size_t r, const uint8_t* sfl,
+__device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3], const R* su, const uint8_t* sfl,
                                          int center, const int (&lstride)[3], int comp, int axis) {
-    const int n[3] = {p.g.nx, p.g.ny, p.g.nz};
-    const size_t rstride[3] = {(size_t)p.g.ny * p.g.nz, (size_t)p.g.nz, 1};
-    const size_t ncell = (size_t)p.g.nx * p.g.ny * p.g.nz;
-    auto U = [&](size_t rr) { return p.u_layout == LBM_U_XYZD ? p.vel[rr * D + comp] : p.vel[comp * ncell + rr]; };
-    const R uc = U(r);
+    const R uc = su[center * D + comp];
     R left_grad = 0, left_dx = 0, right_grad = 0, right_dx = 0;
     const bool left_ok = pos[axis] - 1 >= 0;
     if (left_ok) {
-        const R ua = U(r - rstride[axis]);
-        const R gdx = sfl[center - lstride[axis]] == LBM_FLAG_SOLID ? (R)0.5 : (R)1;
+        const int c = center - lstride[axis];
+        const R ua = su[c * D + comp];
+        const R gdx = sfl[c] == LBM_FLAG_SOLID ? (R)0.5 : (R)1;
         left_grad = (uc - ua) * ((R)1 / gdx);
         left_dx = gdx;
     }
     const bool right_ok = pos[axis] + 1 < n[axis];
     if (right_ok) {
-        const R ua = U(r + rstride[axis]);
-        const R gdx = sfl[center + lstride[axis]] == LBM_FLAG_SOLID ? (R)0.5 : (R)1;
+        const int c = center + lstride[axis];
+        const R ua = su[c * D + comp];
+        const R gdx = sfl[c] == LBM_FLAG_SOLID ? (R)0.5 : (R)1;
         right_grad = (ua - uc) * ((R)1 / gdx);
         right_dx = gdx;
     }
@@ -680,6 +680,78 @@ __device__ __forceinline__ R fd_gradient(const MacroParams<R>& p, const int (&po
     const R left_w = (R)1 - left_dx / total;
     const R right_w = (R)1 - left_w;
     return (left_ok || right_ok) ? right_grad * right_w + left_grad * left_w : (R)0;
+}
+
+// Loads the velocities of a tile and its one-node halo (cells outside the grid: 0, never
+// read) from the row-major field into su, in the flag tile's cell order with D values per
+// cell. Consecutive lanes read consecutive addresses (z, then the component innermost for
+// the interleaved layout; z innermost per component plane for the planar one).
+template <class R, int D>
+__device__ __forceinline__ void load_velocity_tile(const MacroParams<R>& p, int bx, int by, int bz, int hx, int hxy,
+                                                   R* su) {
+    const int tz = 1 << p.g.tzb;
+    const int hy = hxy / hx, hz = D == 3 ? tz + 2 : 1;
+    const int hcells = hxy * hz;
+    const size_t ncell = (size_t)p.g.nx * p.g.ny * p.g.nz;
+    const bool planar = p.u_layout != LBM_U_XYZD;
+    for (int e = threadIdx.x; e < hcells * D; e += blockDim.x) {
+        int d, h;
+        if (planar) {
+            d = e / hcells;
+            h = e - d * hcells;
+        } else {
+            h = e / D;
+            d = e - h * D;
+        }
+        const int iz = h % hz, rest = h / hz;
+        const int iy = rest % hy, ix = rest / hy;
+        const int gx = (bx << p.g.txb) + ix - 1, gy = (by << p.g.tyb) + iy - 1;
+        const int gz = D == 3 ? (bz << p.g.tzb) + iz - 1 : 0;
+        R v = 0;
+        if (gx >= 0 && gx < p.g.nx && gy >= 0 && gy < p.g.ny && gz >= 0 && gz < p.g.nz) {
+            const size_t r = ((size_t)gx * p.g.ny + gy) * p.g.nz + gz;
+            v = planar ? p.vel[(size_t)d * ncell + r] : p.vel[r * D + d];
+        }
+        su[((iz * hy + iy) * hx + ix) * D + d] = v;
+    }
+}
+
+// Row-major outputs of one tile. Lanes hold nodes in the tile's column-major order (x
+// fastest), which a row-major (x, y, z) field would receive as 64 scattered words per
+// wave; instead the tile's values go to LDS in row-major order (K values per node, either
+// interleaved per node or one plane per value) and each lane then writes the next
+// element, so a wave writes runs of tz * K (interleaved) or tz (planar) consecutive
+// words. The LDS buffer holds tile_cells * K values.
+template <class R, int K>
+__device__ __forceinline__ void store_rowmajor(const Geo& g, int bx, int by, int bz, int tid, R* lds, const R (&v)[K],
+                                               R* out, bool planar) {
+    const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
+    const int tc = tx * ty * tz;
+    const int lx = tid & (tx - 1), ly = (tid >> g.txb) & (ty - 1), lz = tid >> (g.txb + g.tyb);
+    const int kr = (lx * ty + ly) * tz + lz;
+    __syncthreads();  // the buffer's previous contents have been written out
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[planar ? k * tc + kr : kr * K + k] = v[k];
+    __syncthreads();
+    const size_t ncell = (size_t)g.nx * g.ny * g.nz;
+#pragma unroll
+    for (int pass = 0; pass < K; ++pass) {
+        const int f = tid + pass * tc;
+        const int node = planar ? f - (f / tc) * tc : f / K;
+        const int k = planar ? f / tc : f - (f / K) * K;
+        const int zl = node & (tz - 1), yl = (node >> g.tzb) & (ty - 1), xl = node >> (g.tzb + g.tyb);
+        const size_t r = ((size_t)((bx << g.txb) + xl) * g.ny + ((by << g.tyb) + yl)) * g.nz + ((bz << g.tzb) + zl);
+        out[planar ? (size_t)k * ncell + r : r * K + k] = lds[f];
+    }
+}
+
+// Tile of a macro_kernel workgroup: the launch order (XCD-contiguous) walks tiles with z
+// fastest, so the workgroups that fill one row-major cache line of output run together.
+__device__ __forceinline__ uint32_t macro_tile(const Geo& g) {
+    const uint32_t t = (uint32_t)tile_of_block(g);
+    const uint32_t bz = t % (uint32_t)g.Tz, rest = t / (uint32_t)g.Tz;
+    const uint32_t by = rest % (uint32_t)g.Ty, bx = rest / (uint32_t)g.Ty;
+    return bx + (uint32_t)g.Tx * (by + (uint32_t)g.Ty * bz);
 }
 
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125), the FP64 moment
@@ -694,7 +766,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroPar
     extern __shared__ uint8_t sfl[];
     __shared__ double red[D + 1][16];
     TileCtx<L> t;
-    t.init_pos(p.g, (uint32_t)tile_of_block(p.g));
+    t.init_pos(p.g, macro_tile(p.g));
     const Field<S, BIG> f(p.f, p.f_bytes);
     t.fetch_flags(p.g, p.flags);
     R fv[Q];
@@ -713,12 +785,15 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroPar
         apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
     }
     if (KIND == MK_QCRIT) {
-        const size_t r = ((size_t)t.x * p.g.ny + t.y) * p.g.nz + t.z;
         R qv = 0;
+        R* su = (R*)(sfl + p.su_offset);
+        load_velocity_tile<R, D>(p, t.bx, t.by, t.bz, t.hx, t.hxy, su);
+        __syncthreads();
         if (t.own != LBM_FLAG_SOLID) {
             const int pos[3] = {t.x, t.y, t.z};
+            const int nn[3] = {p.g.nx, p.g.ny, p.g.nz};
             const int ls[3] = {1, t.hx, t.hxy};
-            auto grad = [&](int comp, int axis) { return fd_gradient<R, D>(p, pos, r, sfl, t.center, ls, comp, axis); };
+            auto grad = [&](int comp, int axis) { return fd_gradient<R, D>(pos, nn, su, sfl, t.center, ls, comp, axis); };
             R wsq;
             if (D == 2) {
                 const R w = grad(1, 0) - grad(0, 1);
@@ -741,7 +816,9 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroPar
             pi_neq<L, R>(fn, pi);
             qv = (R)0.25 * wsq - (R)0.5 * strain_norm_sq<L, R>(pi, rho, p.tau);
         }
-        p.qout[r] = qv;
+        __shared__ R qbuf[MaxThreads<Q>::value];
+        const R qa[1] = {qv};
+        store_rowmajor<R, 1>(p.g, t.bx, t.by, t.bz, t.tid, qbuf, qa, p.qout, false);
         return;
     }
     constexpr bool REDUCE = KIND == MK_REDUCE;
@@ -762,14 +839,13 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroPar
             for (int d = 0; d < D; ++d) u[d] = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, d)];
             rho = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, D)];
         }
-        const size_t r = ((size_t)t.x * p.g.ny + t.y) * p.g.nz + t.z;
-        const size_t ncell = (size_t)p.g.nx * p.g.ny * p.g.nz;
-        p.rho[r] = rho;
+        __shared__ R obuf[MaxThreads<Q>::value * D];
+        const R ra[1] = {rho};
+        store_rowmajor<R, 1>(p.g, t.bx, t.by, t.bz, t.tid, obuf, ra, p.rho, false);
+        R ua[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            if (p.u_layout == LBM_U_XYZD) p.u[r * D + d] = u[d];
-            else p.u[(size_t)d * ncell + r] = u[d];
-        }
+        for (int d = 0; d < D; ++d) ua[d] = u[d];
+        store_rowmajor<R, D>(p.g, t.bx, t.by, t.bz, t.tid, obuf, ua, p.u, p.u_layout != LBM_U_XYZD);
         return;
     }
     // Deterministic block reduction in FP64: wave partials by lane order, then waves in order.

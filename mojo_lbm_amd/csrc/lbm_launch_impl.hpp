@@ -105,12 +105,14 @@ template <class L, class S> struct LaunchImpl {
 
     template <int MODE, bool DDF, int KIND>
     static int macro_k(const Handle& h, const MacroParams<R>& p, hipStream_t st) {
+        // the Q-criterion also stages the velocity tile + halo behind the flag tile
+        const size_t lds = KIND == MK_QCRIT ? p.su_offset + h.lds_bytes * L::D * sizeof(R) : h.lds_bytes;
         if (h.big)
             hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, KIND, true>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
-                               h.lds_bytes, st, p);
+                               lds, st, p);
         else
             hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, KIND, false>), dim3(h.g.n_tiles),
-                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
+                               dim3(h.g.tile_cells), lds, st, p);
         return check_launch("macro_kernel");
     }
 
@@ -157,6 +159,9 @@ template <class L, class S> struct LaunchImpl {
         p.qout = (R*)q;
         p.tau = (R)tau;
         p.u_layout = u_layout;
+        p.su_offset = (uint32_t)((h.lds_bytes + 15) & ~(size_t)15);
+        if (p.su_offset + h.lds_bytes * L::D * sizeof(R) > 65536)
+            return set_error(LBM_ERR_UNSUPPORTED, "tile too large for the Q-criterion's LDS velocity tile (64 KiB)");
         return macro_any<MK_QCRIT>(h, p, mode, st);
     }
 

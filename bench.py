@@ -456,7 +456,7 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (every box has one MI355X). "
+        print("bench.py measures one GPU only: the library has no multi-GPU path (the sharded design is documented, not built). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

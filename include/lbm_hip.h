@@ -64,7 +64,7 @@ enum { LBM_U_XYZD = 0, LBM_U_DXYZ = 1 };
 enum {
     LBM_OK = 0,
     LBM_ERR_INVALID = -1,     /* a validation error (bad desc, bad argument) */
-    LBM_ERR_UNSUPPORTED = -2, /* a valid configuration this build does not provide */
+    LBM_ERR_UNSUPPORTED = -2, /* a valid configuration this library does not provide */
     LBM_ERR_HIP = -3,         /* a HIP runtime error */
     LBM_ERR_NO_DEVICE = -4    /* no usable gfx950 device */
 };

@@ -45,7 +45,7 @@ class LBM_Config:
         if not self.collision_op_is_valid():
             raise ValueError(f"Invalid collision operator {collision_op!r}")
         if capture_density or capture_velocity:
-            raise NotImplementedError("capture_density / capture_velocity are not provided by this build")
+            raise NotImplementedError("capture_density / capture_velocity are not provided by this library")
 
     @property
     def les_id(self) -> int:

@@ -67,7 +67,7 @@ def make_desc_like(desc, **kw) -> LbmDesc:
     return d
 
 
-class GpuRun:
+class DeviceRun:
     """Device buffers + handle for one description, driven through the C ABI."""
 
     def __init__(self, desc, f, bc, flags):

@@ -37,7 +37,7 @@ def _batch(cases, lattice, config, tile=None):
 def _single(b, c, f0, tau, steps):
     """Case c of the batch alone through lbm_step_db (same padded grid and tile)."""
     d, flags, bc, nodes, qd = b.export_case(c)
-    run = U.GpuRun(d, f0, bc, flags)
+    run = U.DeviceRun(d, f0, bc, flags)
     if len(nodes):
         run.set_links(nodes, qd)
     for _ in range(steps):

@@ -23,7 +23,7 @@ def test_links_match_oracle(gpu, lat, storage, ddf):
     print(f"case links {lat} storage={storage} ddf={ddf}", flush=True)
     desc, f, bc, flags, nodes, qdist = U.obstacle_case(lat, storage=storage, ddf=ddf, seed=storage * 7 + ddf)
     tau = 0.6
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     run.set_links(nodes, qdist)
     a, b = f.copy(), f.copy()
     for step in range(4):
@@ -45,7 +45,7 @@ def test_links_match_oracle(gpu, lat, storage, ddf):
 
 def test_links_kbc_les(gpu):
     desc, f, bc, flags, nodes, qdist = U.obstacle_case("D3Q19", storage=0, ddf=1, seed=11, collision=2, les=1)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     run.set_links(nodes, qdist)
     a, b = f.copy(), f.copy()
     for _ in range(3):
@@ -60,7 +60,7 @@ def test_links_clear_and_errors(gpu):
     from mojo_lbm_amd import _lib
 
     desc, f, bc, flags, nodes, qdist = U.obstacle_case("D2Q9")
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     with pytest.raises(_lib.LbmError):
         run.link_force()  # no step with links yet
     with pytest.raises(_lib.LbmError):
@@ -70,7 +70,7 @@ def test_links_clear_and_errors(gpu):
     run.link_force()
     # clearing the links gives the plain step back
     _lib.check(run.L.lbm_set_links(run.h.ptr, None, None, 0))
-    run2 = U.GpuRun(desc, run.f_host(), bc, flags)
+    run2 = U.DeviceRun(desc, run.f_host(), bc, flags)
     run.step_db(0.6)
     run2.step_db(0.6)
     assert np.array_equal(run.f_host(), run2.f_host())

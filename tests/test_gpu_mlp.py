@@ -160,7 +160,7 @@ def test_closure_evaluation_matches_oracle(gpu):
 @pytest.mark.parametrize("storage,method", [(0, 0), (0, 1), (1, 0), (2, 0)])
 def test_closure_in_collide_matches_oracle(gpu, storage, method):
     desc, f, bc, flags, P = _closure_case(storage, method, seed=20 + storage + 3 * method)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     _lib.check(run.L.lbm_set_mlp(run.h.ptr, P.ctypes.data, len(P)))
     O.set_mlp(P)
     steps, tau = 3, 0.51
@@ -187,9 +187,9 @@ def test_closure_in_collide_matches_oracle(gpu, storage, method):
 
 def test_closure_outside_range_is_smagorinsky(gpu):
     desc, f, bc, flags, P = _closure_case(0, 0, seed=31, wide=False)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     _lib.check(run.L.lbm_set_mlp(run.h.ptr, P.ctypes.data, len(P)))
-    smag = U.GpuRun(U.make_desc_like(desc, les=1), f, bc, flags)
+    smag = U.DeviceRun(U.make_desc_like(desc, les=1), f, bc, flags)
     for _ in range(3):
         run.step_db(0.51)
         smag.step_db(0.51)
@@ -198,7 +198,7 @@ def test_closure_outside_range_is_smagorinsky(gpu):
 
 def test_closure_errors(gpu):
     desc, f, bc, flags, P = _closure_case(0, 0, seed=32)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     with pytest.raises(_lib.LbmError):
         run.step_db(0.51)  # no parameters yet
     with pytest.raises(_lib.LbmError):

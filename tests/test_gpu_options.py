@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 def test_option_keeps_results(gpu, option, value):
     desc = U.make_desc("D3Q19", (32, 32, 32))
     f, bc, flags = U.perturbed_state(desc, seed=4)
-    a = U.GpuRun(desc, f, bc, flags)
-    b = U.GpuRun(desc, f, bc, flags)
+    a = U.DeviceRun(desc, f, bc, flags)
+    b = U.DeviceRun(desc, f, bc, flags)
     _lib.check(b.L.lbm_set_option(b.h.ptr, option, value))
     for _ in range(5):
         a.step_db(0.6)
@@ -28,7 +28,7 @@ def test_static_flags_follow_a_new_flags_buffer(gpu):
 
     desc = U.make_desc("D3Q19", (16, 16, 16))
     f, bc, flags = U.perturbed_state(desc, seed=8)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     _lib.check(run.L.lbm_set_option(run.h.ptr, _lib.OPT_STATIC_FLAGS, 1))
     run.step_db(0.7)
     # an obstacle appears in a second flags buffer: the summary must be rebuilt for it

@@ -51,7 +51,7 @@ def _tol(store):
 def test_double_buffer_matches_oracle(gpu, lat, store, variant):
     desc, f, bc, flags = _case(lat, store, variant, method=0)
     print(f"case {lat} {store} {variant} db", flush=True)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     steps = 4
     for _ in range(steps):
         run.step_db(TAU)
@@ -67,7 +67,7 @@ def test_double_buffer_matches_oracle(gpu, lat, store, variant):
 def test_esoteric_pull_matches_oracle(gpu, lat, store, variant):
     desc, f, bc, flags = _case(lat, store, variant, method=1)
     print(f"case {lat} {store} {variant} ep", flush=True)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     steps = 5
     even = True
     for _ in range(steps):
@@ -92,7 +92,7 @@ def test_equilibrium_bc_channel(gpu, lat):
     O.set_exterior_wall(desc, flags, bc, "-X", 2, [0.05] + [0.0] * (D - 1), 1.0)
     O.set_exterior_wall(desc, flags, bc, "+X", 2, None, 1.0)
     print(f"case {lat} eq-channel", flush=True)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     for _ in range(6):
         run.step_db(0.8)
     got = run.f_host()
@@ -124,7 +124,7 @@ def test_scattered_obstacles(gpu, lat, n, tile, method):
         bc[U.index_map(desc, desc.D + 1)[x, y, z, :]] = [0.02] * desc.D + [1.0]
     O.set_exterior_wall(desc, flags, bc, "+Y", 1, [0.05] + [0.0] * (desc.D - 1), 1.0)
     print(f"case {lat} {n} {tile} obstacles method={method}", flush=True)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     steps = 4
     if method == 0:
         for _ in range(steps):
@@ -142,7 +142,7 @@ def test_scattered_obstacles(gpu, lat, n, tile, method):
 @pytest.mark.parametrize("lat", list(SIZES))
 def test_macroscopic_and_reduction(gpu, lat, method):
     desc, f, bc, flags = _case(lat, "f32", "srt", method)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     print(f"case {lat} macro method={method}", flush=True)
     if method == 0:
         run.step_db(TAU)

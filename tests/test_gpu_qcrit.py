@@ -36,7 +36,7 @@ def test_q_criterion_matches_oracle(gpu, lat, storage, ddf, method, u_layout):
     desc = U.make_desc(lat, n, storage=storage, ddf=ddf, method=method)
     f, bc, flags = U.perturbed_state(desc, seed=11 * storage + 3 * ddf + method, amp=2e-3)
     tau = 0.58
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     k = 5
     _steps(run, desc, tau, k)
     after_odd = (k % 2 == 1)
@@ -58,7 +58,7 @@ def test_rotation_closed_form(gpu, lat):
     x, y, z = np.meshgrid(np.arange(desc.nx), np.arange(desc.ny), np.arange(desc.nz), indexing="ij")
     r = np.stack([x - 10.3, y - 12.6, z - 7.2], axis=-1).astype(np.float64)
     u = np.cross(np.broadcast_to(omega, r.shape), r)[..., : desc.D].reshape(-1)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     q = run.q_criterion(u, 0.7).reshape(desc.nx, desc.ny, desc.nz)
     inner = tuple(slice(1, s - 1) if s > 1 else slice(None) for s in (desc.nx, desc.ny, desc.nz))
     np.testing.assert_allclose(q[inner], float(omega @ omega), rtol=1e-9)
@@ -68,7 +68,7 @@ def test_rotation_closed_form(gpu, lat):
 def test_q_criterion_with_links(gpu, lat, storage):
     desc, f, bc, flags, nodes, qdist = U.obstacle_case(lat, storage=storage, ddf=1, seed=5)
     tau = 0.6
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     run.set_links(nodes, qdist)
     for _ in range(3):
         run.step_db(tau)
@@ -83,7 +83,7 @@ def test_q_criterion_errors(gpu):
 
     desc = U.make_desc("D2Q9", (32, 32, 1))
     f, bc, flags = U.perturbed_state(desc)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     u = np.zeros(32 * 32 * 2, np.float32)
     with pytest.raises(RuntimeError, match="u_layout"):
         run.q_criterion(u, 0.6, u_layout=7)

@@ -55,7 +55,7 @@ def test_checkpoint_resume_is_exact(gpu, tmp_path, method):
 
     desc = U.make_desc("D3Q19", (32, 32, 32), method=method, storage=1, ddf=1)
     f, bc, flags = _state(desc, 2)
-    run = U.GpuRun(desc, f, bc, flags)
+    run = U.DeviceRun(desc, f, bc, flags)
     step = (lambda k: run.step_db(0.6)) if method == 0 else (lambda k: run.step_ep(0.6, k % 2 == 0))
     for k in range(7):
         step(k)
@@ -65,7 +65,7 @@ def test_checkpoint_resume_is_exact(gpu, tmp_path, method):
         step(k)
     ref = run.f_host()
     d2, f2, bc2, fl2, hdr = snapshot.load(p)
-    run2 = U.GpuRun(d2, f2.cpu().numpy().view(f.dtype), bc2.cpu().numpy().view(bc.dtype), fl2.cpu().numpy())
+    run2 = U.DeviceRun(d2, f2.cpu().numpy().view(f.dtype), bc2.cpu().numpy().view(bc.dtype), fl2.cpu().numpy())
     step2 = (lambda k: run2.step_db(0.6)) if method == 0 else (lambda k: run2.step_ep(0.6, k % 2 == 0))
     for k in range(hdr["step"], 12):
         step2(k)

@@ -359,7 +359,8 @@ int lbm_set_mlp(lbm_handle* hh, const float* params, int32_t n) {
     for (int32_t i = 0; i < n; ++i)
         if (!std::isfinite(params[i])) return set_error(LBM_ERR_INVALID, "non-finite closure parameter");
     // the float block, then W1^T and W2^T as bf16 MFMA A operands (lane l: row l mod 32,
-    // K elements 8 (l / 32) + i; for W2 the K index of step s is hidden unit mfma_row(g, 8 s + i))
+    // K elements 8 (l / 32) + i; for W2 the K index of step s is hidden unit mfma_row(g, 8 s + i)),
+    // then each wave half's b1, b2, w3 values in result-register order
     std::vector<uint8_t> blob(MLP_DEVICE_BYTES, 0);
     std::memcpy(blob.data(), params, MLP_TOTAL * sizeof(float));
     auto bf16 = [](float v) -> uint16_t {  // round to nearest even
@@ -377,6 +378,13 @@ int lbm_set_mlp(lbm_handle* hh, const float* params, int32_t n) {
                 const int kin = mfma_row(g, 8 * s2 + i);
                 a2[s2 * 512 + l * 8 + i] = bf16(params[MLP_W2 + kin * MLP_H + m]);
             }
+        }
+    float* rows = (float*)(blob.data() + MLP_PACK_ROWS);
+    for (int g = 0; g < 2; ++g)
+        for (int j = 0; j < 16; ++j) {
+            rows[g * 48 + j] = params[MLP_B1 + mfma_row(g, j)];
+            rows[g * 48 + 16 + j] = params[MLP_B2 + mfma_row(g, j)];
+            rows[g * 48 + 32 + j] = params[MLP_W3 + mfma_row(g, j)];
         }
     if (!h.mlp && hipMalloc((void**)&h.mlp, MLP_DEVICE_BYTES) != hipSuccess)
         return set_error(LBM_ERR_HIP, "hipMalloc of the closure parameters failed");

@@ -199,7 +199,10 @@ __device__ __forceinline__ float softplus_fast(float x) {
 // inputs zero-padded) and W2^T as two 32 x 16 K-steps. Built by lbm_set_mlp.
 constexpr int MLP_PACK_A1 = 5888;                       // bytes
 constexpr int MLP_PACK_A2 = MLP_PACK_A1 + 64 * 16;      // two steps of 64 lanes x 16 bytes
-constexpr int MLP_DEVICE_BYTES = MLP_PACK_A2 + 2 * 64 * 16;
+// Then, per wave half g, the 48 per-row values a lane of that half needs, in the order of its
+// MFMA result registers j: b1[mfma_row(g, j)], b2[mfma_row(g, j)], w3[mfma_row(g, j)] (16 each).
+constexpr int MLP_PACK_ROWS = MLP_PACK_A2 + 2 * 64 * 16;
+constexpr int MLP_DEVICE_BYTES = MLP_PACK_ROWS + 2 * 48 * 4;
 
 // Row of the 32 x 32 MFMA result held in register j of a lane in wave half g: the result
 // column is the lane index mod 32, rows 8 (j / 4) + 4 g + j % 4.
@@ -211,17 +214,23 @@ typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 // The closure's nu_t for this lane's node, evaluated by the whole wave: the nodes of lanes
 // 0-31 and 32-63 form two 32-column blocks; each hidden layer is W^T (32 x K) times the
-// block's inputs (K x 32) on the matrix cores in bf16 with FP32 accumulation. Every lane of
-// the wave must be active. `inside` is false when a raw feature lies outside the training range.
+// block's inputs (K x 32) on the matrix cores in bf16 with FP32 accumulation, the biases as the
+// accumulator inputs. Every lane of the wave must be active. `inside` is false when a raw
+// feature lies outside the training range.
 typedef const float __attribute__((address_space(4)))* MlpConstPtr;
 
-// P[i] for this lane's wave half: both candidates are loaded at uniform addresses and the choice
-// is a vector select (readfirstlane keeps the compiler from turning the select of two loads
-// into one load at a lane-dependent address in the constant address space).
-__device__ __forceinline__ float mlp_half(MlpConstPtr P, int i_lo, int i_hi, bool hi) {
-    const float lo = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, P[i_lo])));
-    const float hv = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, P[i_hi])));
-    return hi ? hv : lo;
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// 16 consecutive floats of this lane's wave-half row table (four 16-byte vector loads).
+__device__ __forceinline__ f32x16_t mlp_rows16(const f32x4_t* rows, int group) {
+    f32x16_t v;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const f32x4_t r = rows[group * 4 + q];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[4 * q + e] = r[e];
+    }
+    return v;
 }
 
 __device__ __forceinline__ float mlp_nu_wave(const float* __restrict__ Pg, const float (&x)[MLP_IN], bool& inside) {
@@ -257,15 +266,18 @@ __device__ __forceinline__ float mlp_nu_wave(const float* __restrict__ Pg, const
     const bf16x8_t a1 = *(const bf16x8_t*)(pack + MLP_PACK_A1 + lane * 16);
     const bf16x8_t a20 = *(const bf16x8_t*)(pack + MLP_PACK_A2 + lane * 16);
     const bf16x8_t a21 = *(const bf16x8_t*)(pack + MLP_PACK_A2 + 1024 + lane * 16);
+    const f32x4_t* rows_base = (const f32x4_t*)(pack + MLP_PACK_ROWS + (hi ? 192 : 0));
     // one 32-node block at a time keeps a single block's accumulators live
     float part[2];
 #pragma unroll 1
     for (int blk = 0; blk < 2; ++blk) {
         const u32x4_t bin = blk ? b1 : b0;
-        f32x16_t c;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) c[j] = mlp_half(P, MLP_B1 + mfma_row(0, j), MLP_B1 + mfma_row(1, j), hi);
-        const f32x16_t d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, __builtin_bit_cast(bf16x8_t, bin), c, 0, 0, 0);
+        // reload the row tables in each block (cache hits) rather than keep 48 values live
+        uintptr_t rp = (uintptr_t)rows_base;
+        __asm__ volatile("" : "+v"(rp));
+        const f32x4_t* rows = (const f32x4_t*)rp;
+        const f32x16_t d1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, __builtin_bit_cast(bf16x8_t, bin),
+                                                                    mlp_rows16(rows, 0), 0, 0, 0);
         // layer 2: the K index of step s, half g, element i is hidden unit mfma_row(g, 8 s + i),
         // which is exactly where layer 1 left it; W2 is packed to match
         bf16x8_t h0, h1;
@@ -274,17 +286,13 @@ __device__ __forceinline__ float mlp_nu_wave(const float* __restrict__ Pg, const
             h0[i] = (__bf16)silu_fast(d1[i]);
             h1[i] = (__bf16)silu_fast(d1[8 + i]);
         }
-#pragma unroll
-        for (int j = 0; j < 16; ++j) c[j] = mlp_half(P, MLP_B2 + mfma_row(0, j), MLP_B2 + mfma_row(1, j), hi);
-        f32x16_t d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a20, h0, c, 0, 0, 0);
+        f32x16_t d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a20, h0, mlp_rows16(rows, 1), 0, 0, 0);
         d2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a21, h1, d2, 0, 0, 0);
         // layer 3: partial dot product over this lane's 16 rows
+        const f32x16_t w3 = mlp_rows16(rows, 2);
         float sacc = 0.0f;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const float w = mlp_half(P, MLP_W3 + mfma_row(0, j), MLP_W3 + mfma_row(1, j), hi);
-            sacc = __builtin_fmaf(w, silu_fast(d2[j]), sacc);
-        }
+        for (int j = 0; j < 16; ++j) sacc = __builtin_fmaf(w3[j], silu_fast(d2[j]), sacc);
         part[blk] = sacc;
     }
     const float sa = part[0], sb = part[1];

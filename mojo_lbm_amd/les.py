@@ -284,8 +284,13 @@ def _coarse_u(c: Cavity):
     return u.view(N_COARSE, N_COARSE, N_COARSE, 3)[1:-1, 1:-1, 1:-1].double()
 
 
-def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=print):
-    """A-posteriori comparison of coarse closures against the filtered held-out fine run."""
+def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=print, stats_steps=0,
+             sample_every=25):
+    """A-posteriori comparison of coarse closures against the filtered held-out fine run: velocity
+    rel-L2 at checkpoints, then (stats_steps > 0) the rel-L2 of the time-mean velocity and of the
+    rms velocity fluctuation over stats_steps further coarse steps sampled every sample_every steps
+    (instantaneous fields decorrelate from the fine run within about a thousand coarse steps; the
+    statistics do not)."""
     torch = _torch()
     t0 = time.time()
     fine = Cavity(N_FINE, FINE_LO, tau_for(N_FINE), storage=_lib.STORE_F16C, collision=collision, seed=seed)
@@ -328,6 +333,36 @@ def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=
     for name, acc in sums.items():
         e = float(torch.linalg.vector_norm(acc - ref_sum) / torch.linalg.vector_norm(ref_sum))
         mean_err[name] = e if math.isfinite(e) else None
+    stats = None
+    if stats_steps > 0:
+        n = 0
+        acc = {name: [0.0, 0.0] for name in ["reference", *coarse]}
+
+        def add(name, u):
+            a = acc[name]
+            a[0] = a[0] + u
+            a[1] = a[1] + u * u
+
+        for k in range(stats_steps // sample_every):
+            fine.step(FACTOR * sample_every)
+            add("reference", filtered(fine).view(N_COARSE, N_COARSE, N_COARSE, 10)[1:-1, 1:-1, 1:-1, 1:4].double())
+            for name, c in coarse.items():
+                c.step(sample_every)
+                add(name, _coarse_u(c))
+            n += 1
+            if (k + 1) % 40 == 0:
+                log(f"# eval statistics: {n} samples, {time.time() - t0:.0f} s")
+        mean_r = acc["reference"][0] / n
+        rms_r = (acc["reference"][1] / n - mean_r * mean_r).clamp_min(0).sqrt()
+        stats = {"coarse_steps": stats_steps, "samples": n, "rel_l2_mean_velocity": {}, "rel_l2_rms_fluctuation": {}}
+        for name in coarse:
+            mean_c = acc[name][0] / n
+            rms_c = (acc[name][1] / n - mean_c * mean_c).clamp_min(0).sqrt()
+            e1 = float(torch.linalg.vector_norm(mean_c - mean_r) / torch.linalg.vector_norm(mean_r))
+            e2 = float(torch.linalg.vector_norm(rms_c - rms_r) / torch.linalg.vector_norm(rms_r))
+            stats["rel_l2_mean_velocity"][name] = e1 if math.isfinite(e1) else None
+            stats["rel_l2_rms_fluctuation"][name] = e2 if math.isfinite(e2) else None
+        log(f"# eval statistics {stats}")
     times = {}
     for name in ("smagorinsky", "learned", "none"):
         c = coarse[name]
@@ -339,7 +374,7 @@ def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=
         end.record()
         torch.cuda.synchronize()
         times[name] = start.elapsed_time(end) / 200
-    return {"seed": seed, "fine_steps_spinup": spinup, "rel_l2": rows, "rel_l2_time_mean": mean_err,
+    return {"seed": seed, "fine_steps_spinup": spinup, "rel_l2": rows, "rel_l2_time_mean": mean_err, "statistics": stats,
             "coarse_ms_per_step": times,
             "learned_over_smagorinsky_step_time": times["learned"] / times["smagorinsky"],
             "seconds": round(time.time() - t0, 1)}
@@ -365,6 +400,8 @@ def main(argv=None):
     e.add_argument("--horizon", type=int, default=2000, help="coarse steps compared")
     e.add_argument("--collision", default="SRT")
     e.add_argument("--coarse", type=int, default=128)
+    e.add_argument("--stats-steps", type=int, default=0, help="coarse steps of time statistics after the checkpoints")
+    e.add_argument("--sample-every", type=int, default=25)
     e.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     set_sizes(a.coarse)
@@ -396,7 +433,8 @@ def main(argv=None):
         saved = json.load(open(a.params))
         P = np.asarray(saved["params"], np.float32)
         coll = saved.get("config", {}).get("collision", a.collision)
-        res = evaluate(P, a.seed, a.spinup, a.horizon, collision=coll, log=log)
+        res = evaluate(P, a.seed, a.spinup, a.horizon, collision=coll, log=log, stats_steps=a.stats_steps,
+                       sample_every=a.sample_every)
         res["fine_collision"] = coll
         with open(a.out, "w") as fh:
             json.dump(res, fh)

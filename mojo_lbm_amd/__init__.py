@@ -17,7 +17,7 @@ __all__ = [
     "TiledLayouts", "Lattice", "get_D2Q9", "get_D3Q19", "get_D3Q27", "UnitSystem", "tau_from_re",
     "Solver", "Assembly", "ContextTileTensor", "set_exterior_walls", "set_exterior_walls_with_func",
     "initialize_fluid_at_rest", "initialize_equilibrium", "initialize_f_from_func", "calculate_rho_and_velocity",
-    "calculate_Q_criterion", "reduce_moments", "OutputRequest",
+    "calculate_Q_criterion", "reduce_moments", "OutputRequest", "check_health",
 ]
 
 
@@ -36,7 +36,8 @@ def __getattr__(name):
                 "initialize_equilibrium", "initialize_f_from_func"):
         from . import preprocess
         return getattr(preprocess, name)
-    if name in ("calculate_rho_and_velocity", "calculate_Q_criterion", "reduce_moments", "OutputRequest"):
+    if name in ("calculate_rho_and_velocity", "calculate_Q_criterion", "reduce_moments", "OutputRequest",
+                "check_health"):
         from . import output
         return getattr(output, name)
     if name == "snapshot":

@@ -113,3 +113,16 @@ class LBM_Grid:
 
     def get_UnitSystem(self, U_phys, U_lattice, L_phys, kinematic_viscosity, density=1.0) -> UnitSystem:
         return UnitSystem(U_phys, U_lattice, L_phys, L_phys / self.dx, density, kinematic_viscosity, D=self.D)
+
+    def get_UnitSystem_with_tau(self, U_phys, tau, L_phys, kinematic_viscosity, density=1.0, *, min_u=0.001,
+                                max_u=0.1) -> UnitSystem:
+        """grid.mojo:238-277: the lattice velocity that gives relaxation time tau at this Re;
+        raises when it falls outside [min_u, max_u]."""
+        L_lattice = L_phys / self.dx
+        v_lat = (tau - 0.5) * (1 / 3.0)
+        re = U_phys * L_phys / kinematic_viscosity
+        U_lattice = v_lat * re / L_lattice
+        if U_lattice < min_u or U_lattice > max_u:
+            raise ValueError(f"Target Tau of {tau} does not satisfy bounds. Caclulated u lattice = {U_lattice} "
+                             f"which is outside the given bounds of [{min_u} {max_u}]")
+        return UnitSystem(U_phys, U_lattice, L_phys, L_lattice, density, kinematic_viscosity, D=self.D)

@@ -61,6 +61,19 @@ def reduce_moments(solver, assembly, after_odd_step: bool = True) -> np.ndarray:
     return out.cpu().numpy()
 
 
+def check_health(solver, assembly, after_odd_step: bool = True, mass0: float | None = None,
+                 max_mass_drift: float = 1e-3) -> np.ndarray:
+    """Divergence watchdog (the reference has none): the FP64 totals of reduce_moments must be
+    finite and, when mass0 is given, the total mass within max_mass_drift (relative) of it.
+    Returns the totals; raises FloatingPointError otherwise."""
+    tot = reduce_moments(solver, assembly, after_odd_step)
+    if not np.all(np.isfinite(tot)):
+        raise FloatingPointError(f"the run diverged: mass / momentum totals {tot}")
+    if mass0 is not None and abs(tot[0] - mass0) > max_mass_drift * abs(mass0):
+        raise FloatingPointError(f"total mass drifted from {mass0} to {tot[0]} (more than {max_mass_drift:g})")
+    return tot
+
+
 def _current_f(solver, assembly, after_odd_step: bool):
     if solver.lbm_method == LBM_method.DOUBLE_BUFFER:
         return assembly.f.gpu() if after_odd_step else assembly.f2.gpu()

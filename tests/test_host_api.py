@@ -120,3 +120,24 @@ def test_moving_wall_requires_config_toggle():
     with pytest.raises(ValueError, match="include_moving_boundary"):
         lbm.set_exterior_walls(g, cfg, _HostOnly(np.zeros(16 ** 3, np.uint8)),
                                _HostOnly(np.zeros(16 ** 3 * 4, np.float32)), "+Y", 1, [0.1, 0, 0], 1.0)
+
+
+def test_unit_system_with_tau_bounds():
+    """grid.mojo:238-277: u_lat = (tau - 1/2) / 3 * Re / L_lat, rejected outside [0.001, 0.1]."""
+    g = lbm.LBM_Grid(lbm.get_D3Q19(), 64, 64, 64, 8, dx=1.0 / 64)
+    us = g.get_UnitSystem_with_tau(1.0, 0.56, 1.0, 1e-2)
+    assert math.isclose(us.U.lattice, 0.06 / 3 * 100 / 64)
+    assert math.isclose(us.tau, 0.56)
+    with pytest.raises(ValueError, match="does not satisfy bounds"):
+        g.get_UnitSystem_with_tau(1.0, 0.9, 1.0, 1e-2)
+
+
+def test_assembly_method_mismatch():
+    """assembly.mojo:100-164: asking an Assembly for the other method's buffers raises."""
+    g = lbm.LBM_Grid(lbm.get_D2Q9(), 32, 32, 1, 16)
+    ep = lbm.Assembly(g, lbm.EsotericPullConfig())
+    with pytest.raises(RuntimeError, match="esoteric pull"):
+        ep.get_gpu_tensors_for_double_buffer()
+    db = lbm.Assembly(g, lbm.DoubleBufferConfig())
+    with pytest.raises(RuntimeError, match="double buffer"):
+        db.get_gpu_tensors_for_esoteric_pull()

@@ -115,3 +115,30 @@ def test_assembly_checkpoint_resume(gpu, tmp_path, method_name):
     for k in range(9, 14):
         step(solver2, asm2, k)
     assert np.array_equal(current(asm2, 14).cpu(), ref)
+
+
+@pytest.mark.gpu
+def test_check_health_flags_divergence(gpu):
+    """check_health: finite totals pass, a mass drift beyond the bound or a NaN population raises."""
+    import torch
+
+    import mojo_lbm_amd as lbm
+
+    grid = lbm.LBM_Grid(lbm.get_D2Q9(), 32, 32, 1, 16)
+    config = lbm.LBM_Config(lbm.DOUBLE_BUFFER)
+    solver = lbm.Solver(grid, config)
+    asm = lbm.Assembly(grid, config)
+    lbm.initialize_fluid_at_rest(grid, config, asm.f, solver)
+    lbm.initialize_fluid_at_rest(grid, config, asm.f2, solver)
+    lbm.set_exterior_walls(grid, config, asm.flags, asm.bc, "+Y", lbm.Flags.SOLID, [0.1, 0.0], 1.0)
+    for k in range(4):
+        (solver.even_step if k % 2 == 0 else solver.odd_step)(asm, 0.7)
+    tot = lbm.check_health(solver, asm, True)
+    lbm.check_health(solver, asm, True, mass0=float(tot[0]))
+    with pytest.raises(FloatingPointError, match="drifted"):
+        lbm.check_health(solver, asm, True, mass0=float(tot[0]) * 1.01)
+    f = asm.f.gpu().view(torch.float32)
+    f[123] = float("nan")
+    asm.f.mark_device_written()
+    with pytest.raises(FloatingPointError, match="diverged"):
+        lbm.check_health(solver, asm, True)

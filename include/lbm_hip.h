@@ -295,9 +295,12 @@ int lbm_init_f_from_fields(lbm_handle* h, void* f, double rho, const void* u, co
  *                      flag summary is rebuilt only when either pointer changes
  *                      (or this option is set again), not before every step.
  *                      0 (default) = rebuilt before every step.
+ *   LBM_OPT_ADDR64:    1 = use the 64-bit addressing kernels (which buffers of 2 GiB and
+ *                      more always use) for any size, so they can be checked on small grids;
+ *                      0 (default) = only for buffers of 2 GiB and more.
  * Returns LBM_ERR_INVALID for an unknown option or value.
  */
-enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2 };
+enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2, LBM_OPT_ADDR64 = 3 };
 int lbm_set_option(lbm_handle* h, int option, int value);
 
 /* Device memory helpers for hosts without their own GPU runtime (the Mojo host

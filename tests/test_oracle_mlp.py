@@ -66,3 +66,24 @@ def test_closure_range_fallback():
 def test_softplus_and_silu_values(a):
     assert M._softplus(np.array(a)) == pytest.approx(np.log1p(np.exp(a)) if a <= 20 else a, rel=1e-12)
     assert M._silu(np.array(a)) == pytest.approx(a / (1 + np.exp(-a)), rel=1e-12)
+
+
+def test_oracle_closure_matches_fp64_mlp():
+    """The C++ oracle's closure (the one its collide uses: bf16-rounded inputs, weights of the two hidden
+    layers and first hidden activations, as the matrix cores see them) agrees with the FP64 MLP to bf16
+    accuracy; a transposed weight matrix or a wrong activation would miss by O(1)."""
+    from oracle import oracle as O
+
+    P = M.random_params(11, nu_scale=1.0)
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((400, 9)).astype(np.float32)
+    got = O.mlp_closure(P, X)
+    ref = M.closure_nu(P, X, np.full(len(X), -1.0))
+    assert np.all(got > 0)
+    np.testing.assert_allclose(got, ref, rtol=3e-2)
+    assert np.median(np.abs(got / ref - 1)) < 5e-3
+    P2 = P.copy()
+    P2[M.HI:M.NUS] = 0.5  # most samples leave the training range
+    out = O.mlp_closure(P2, X)
+    inside = np.all(X <= 0.5, axis=1)
+    assert np.all(out[~inside] == -1.0) and np.all(out[inside] > 0)

@@ -760,7 +760,8 @@ __device__ __forceinline__ uint32_t macro_tile(const Geo& g) {
 // finite differences of the velocity field and S from the re-streamed populations.
 // Solid nodes (raw flag == SOLID) report bc in the fields and 0 as Q.
 template <class L, class S, int MODE, bool DDF, int KIND, bool BIG>
-__global__ void __launch_bounds__(MaxThreads<L::Q>::value) macro_kernel(MacroParams<typename S::R> p) {
+__global__ void __launch_bounds__(MaxThreads<L::Q>::value) __attribute__((amdgpu_waves_per_eu(6, 8)))
+    macro_kernel(MacroParams<typename S::R> p) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
     extern __shared__ uint8_t sfl[];

@@ -1,7 +1,7 @@
 """Batched cases (one launch per step over many grids, hipGraph replays) against the
 same cases run one at a time through lbm_step_db: populations bit-identical, forces
 to FP64 summation order. Covers graph replays plus a remainder, the persistent grid,
-a 3-D batch with KBC + LES, replaced states, the autotuner and the API's errors."""
+3-D batches with KBC / TRT / RLBM + LES, replaced states, the autotuner and the API's errors."""
 import numpy as np
 import pytest
 
@@ -70,7 +70,8 @@ def test_batch_matches_single_runs(gpu, grid_blocks, graph_steps, steps):
         assert np.allclose(forces[i], fref, rtol=1e-12, atol=1e-15)
 
 
-def test_batch_3d_kbc_les_from_perturbed_states(gpu):
+@pytest.mark.parametrize("collision", ["KBC", "TRT", "RLBM"])
+def test_batch_3d_les_from_perturbed_states(gpu, collision):
     import torch
 
     from mojo_lbm_amd import Flags, get_D3Q19
@@ -80,7 +81,8 @@ def test_batch_3d_kbc_les_from_perturbed_states(gpu):
     lat = get_D3Q19()
     sizes = [(24, 16, 16), (16, 16, 8), (40, 24, 16)]
     taus = [0.52, 0.6, 0.55]
-    cfg = _config(collision_op="KBC", LES=True, DDF_shift=True)
+    print(f"case batch 3-D {collision} + LES", flush=True)
+    cfg = _config(collision_op=collision, LES=True, DDF_shift=True)
     b = CaseBatch(lat, sizes, taus, cfg, tile=(8, 8, 8))
     rng = np.random.default_rng(5)
     for i, n in enumerate(sizes):

@@ -1,7 +1,7 @@
 """Taylor-Green vortex decay on the GPU: a periodic 2-D vortex set up with initialize_f_from_func
 (equilibrium plus the non-equilibrium gradient term) decays as exp(-2 nu k^2 t) with the lattice
-viscosity nu = (tau - 1/2) / 3 of SRT, TRT and RLBM (and in the x-y plane of a D3Q19 box); the
-measured decay rate of the velocity amplitude must match to 0.5 %."""
+viscosity nu = (tau - 1/2) / 3 of SRT, TRT, RLBM and KBC (and in the x-y plane of a D3Q19 box); the
+measured decay rate of the velocity amplitude must match to 0.1 % (measured: within 3e-4)."""
 import numpy as np
 import pytest
 
@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("lat,collision,f16c", [("D2Q9", "SRT", False), ("D2Q9", "TRT", False),
-                                                 ("D2Q9", "RLBM", False), ("D2Q9", "SRT", True),
+                                                 ("D2Q9", "RLBM", False), ("D2Q9", "KBC", False),
+                                                 ("D2Q9", "SRT", True),
                                                  ("D3Q19", "SRT", False)])
 def test_taylor_green_decay(gpu, lat, collision, f16c):
     print(f"case taylor-green {lat} {collision} f16c={f16c}", flush=True)
@@ -50,4 +51,4 @@ def test_taylor_green_decay(gpu, lat, collision, f16c):
     rate = np.log(amps[0] / amps[1]) / steps
     expect = 2 * nu * k * k
     print(f"decay rate {rate:.6e}, 2 nu k^2 = {expect:.6e}, relative error {rate / expect - 1:+.2e}", flush=True)
-    assert abs(rate / expect - 1) < 5e-3, (rate, expect)
+    assert abs(rate / expect - 1) < 1e-3, (rate, expect)

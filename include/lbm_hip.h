@@ -298,9 +298,16 @@ int lbm_init_f_from_fields(lbm_handle* h, void* f, double rho, const void* u, co
  *   LBM_OPT_ADDR64:    1 = use the 64-bit addressing kernels (which buffers of 2 GiB and
  *                      more always use) for any size, so they can be checked on small grids;
  *                      0 (default) = only for buffers of 2 GiB and more.
+ *   LBM_OPT_STEP_LANES: lanes of one lbm_step_db / lbm_step_ep workgroup: a power of
+ *                      two >= 64 that is a whole number of the tile's z layers, so
+ *                      2^k workgroups share a tile, each with a z slab of it (the
+ *                      (x,y,z,q) layout is unchanged); 0 or tile_cells = one
+ *                      workgroup per tile. Default: 256 for Float16C storage on
+ *                      3-D tiles of more than 256 nodes, else one workgroup per
+ *                      tile. Results are identical for every value.
  * Returns LBM_ERR_INVALID for an unknown option or value.
  */
-enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2, LBM_OPT_ADDR64 = 3 };
+enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2, LBM_OPT_ADDR64 = 3, LBM_OPT_STEP_LANES = 4 };
 int lbm_set_option(lbm_handle* h, int option, int value);
 
 /* Device memory helpers for hosts without their own GPU runtime (the Mojo host

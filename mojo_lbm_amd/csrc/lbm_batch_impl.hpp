@@ -31,7 +31,8 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
                 p.f_in = p.f_out;
                 p.f_out = const_cast<void*>(a);
             }
-            step_tile<L, S, MODE_DB, COLL, LES, DDF, false, LINKS>(p, gt - cases[c].tile_begin, gt, sfl);
+            step_tile<L, S, MODE_DB, COLL, LES, DDF, false, LINKS>(p, gt - cases[c].tile_begin, gt, sfl, 0,
+                                                                    p.g.tzb);
         }
         if (b + gridDim.x < n_slots) __syncthreads();  // the next tile reuses the LDS flag tile
     }

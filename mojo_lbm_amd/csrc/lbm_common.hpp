@@ -81,19 +81,20 @@ template <class L, class R> __host__ __device__ __forceinline__ constexpr R weig
 // library keeps (IEEE mode, no flush). The code is sign-extended first (a signed
 // 16-bit load): shifted left by 12 its sign lands on bits 27..31, and one mask
 // keeps bit 31 and the magnitude.
-__device__ __forceinline__ float f16c_decode(uint32_t h) {
-    const uint32_t hs = (uint32_t)(int32_t)(int16_t)(uint16_t)h;
-    return __uint_as_float((hs << 12) & 0x87FFFFFFu) * 0x1p112f;
+__device__ __forceinline__ float f16c_decode_sext(int32_t hs) {
+    return __uint_as_float(((uint32_t)hs << 12) & 0x87FFFFFFu) * 0x1p112f;
 }
+__device__ __forceinline__ float f16c_decode(uint32_t h) { return f16c_decode_sext((int16_t)(uint16_t)h); }
 
 // Encode, for |x| = a (fp32 bits without the sign):
 //   normal range: adding half a code ulp (2^11 in fp32 bits) and truncating 12
 //   bits rounds half away from zero, and subtracting 112 << 11 rebiases the
 //   exponent: (a + 2^11 - (112 << 23)) >> 12, taken signed, is negative below
 //   2^-15 and at most the subnormal code up to 2^-14;
-//   subnormal range: with |x| clamped to 2^-14, t = |x| 2^26 is exact and below
-//   4097, floor(t) is exact, and (floor(t) + 1) >> 1 rounds |x| 2^25 half up
-//   (at most 2048, the first normal code).
+//   subnormal range: with |x| clamped to 2^-14, u = |x| 2^25 is exact and at
+//   most 2048 (the first normal code), and the code rounds it half up:
+//   floor(u + 1/2), which v_cvt_rpi_i32_f32 computes without rounding the sum
+//   (the same as (floor(2u) + 1) >> 1).
 // The code is the larger of the two, clamped to 0x7FFF, which also saturates
 // magnitudes of 2 - 2^-12 and above, infinities and NaN. Checked against the
 // format definition on all 2^32 inputs.
@@ -101,8 +102,9 @@ __device__ __forceinline__ uint32_t f16c_encode(float x) {
     const uint32_t bits = __float_as_uint(x);
     const uint32_t a = bits & 0x7FFFFFFFu;
     const int32_t normal = (int32_t)(a - 0x37FFF800u) >> 12;
-    const float t = __uint_as_float(min(a, 0x38800000u)) * 0x1p26f;
-    const int32_t sub = (int32_t)(((uint32_t)t + 1u) >> 1);
+    const float u = __uint_as_float(min(a, 0x38800000u)) * 0x1p25f;
+    int32_t sub;
+    __asm__("v_cvt_rpi_i32_f32 %0, %1" : "=v"(sub) : "v"(u));
     int32_t code;  // min(max(normal, sub), 0x7FFF) in one instruction (sub <= 0x7FFF)
     __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(code) : "v"(normal), "v"(sub), "s"(0x7FFF));
     return ((bits >> 16) & 0x8000u) | (uint32_t)code;
@@ -144,16 +146,19 @@ struct Geo {
     int tile_cells;       // tx * ty * tz
     int n_tiles;
     int xcd_chunk;        // tiles per XCD when the XCD-contiguous remap is on, else 0
+    int wzb;              // log2 of the z layers one step workgroup covers (tzb: a whole tile)
     uint32_t hx_magic;    // floor(2^32 / (tx + 2)) + 1: n / (tx + 2) == umulhi(n, hx_magic) for n < 2^16
     uint32_t hxy_magic;   // the same for (tx + 2) * (ty + 2)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X. With the
 // remap, XCD k walks a contiguous run of tiles, so the x- and y-neighbour tiles
-// whose halo cache lines a tile shares are resident in the same L2.
-__device__ __forceinline__ int tile_of_block(const Geo& g) {
+// whose halo cache lines a tile shares are resident in the same L2. With 2^s
+// workgroups per tile the result is the work unit tile * 2^s + part, and the
+// parts of one tile stay on one XCD.
+__device__ __forceinline__ int tile_of_block(const Geo& g, int s = 0) {
     const int b = blockIdx.x;
-    return g.xcd_chunk ? (b & 7) * g.xcd_chunk + (b >> 3) : b;
+    return g.xcd_chunk ? (b & 7) * (g.xcd_chunk << s) + (b >> 3) : b;
 }
 
 }  // namespace lbm

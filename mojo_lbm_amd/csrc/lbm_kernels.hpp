@@ -104,6 +104,10 @@ template <> struct Bits<StoreF16C> {
     __device__ __forceinline__ static float get(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
         return f16c_decode(__builtin_amdgcn_raw_buffer_load_b16(r, v, s, 0));
     }
+    // The code sign-extended (buffer_load_sshort once the value passes a register fence).
+    __device__ __forceinline__ static int32_t raw(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return (int16_t)__builtin_amdgcn_raw_buffer_load_b16(r, v, s, 0);
+    }
     __device__ __forceinline__ static void put(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x) {
         __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f16c_encode(x), r, v, s, kStoreNT);
     }
@@ -120,6 +124,27 @@ template <> struct Bits<StoreF64> {
 
 // One population field. ld/st take the lane's element index (node part) and a
 // uniform element offset (the q plane).
+// An empty asm that takes every register of r as input and output: values loaded before it
+// reach it as loaded, so the compiler cannot fold a sign extension into the decode (which
+// leaves the extension to the load), and every load is issued before the first decode.
+#define LBM_RF(i) "+v"(r[i])
+template <int Q> __device__ __forceinline__ void register_fence(int32_t (&r)[Q]);
+template <> __device__ __forceinline__ void register_fence<9>(int32_t (&r)[9]) {
+    __asm__("" : LBM_RF(0), LBM_RF(1), LBM_RF(2), LBM_RF(3), LBM_RF(4), LBM_RF(5), LBM_RF(6), LBM_RF(7), LBM_RF(8));
+}
+template <> __device__ __forceinline__ void register_fence<19>(int32_t (&r)[19]) {
+    __asm__("" : LBM_RF(0), LBM_RF(1), LBM_RF(2), LBM_RF(3), LBM_RF(4), LBM_RF(5), LBM_RF(6), LBM_RF(7), LBM_RF(8),
+            LBM_RF(9), LBM_RF(10), LBM_RF(11), LBM_RF(12), LBM_RF(13), LBM_RF(14), LBM_RF(15), LBM_RF(16),
+            LBM_RF(17), LBM_RF(18));
+}
+template <> __device__ __forceinline__ void register_fence<27>(int32_t (&r)[27]) {
+    __asm__("" : LBM_RF(0), LBM_RF(1), LBM_RF(2), LBM_RF(3), LBM_RF(4), LBM_RF(5), LBM_RF(6), LBM_RF(7), LBM_RF(8),
+            LBM_RF(9), LBM_RF(10), LBM_RF(11), LBM_RF(12), LBM_RF(13), LBM_RF(14), LBM_RF(15), LBM_RF(16),
+            LBM_RF(17), LBM_RF(18), LBM_RF(19), LBM_RF(20), LBM_RF(21), LBM_RF(22), LBM_RF(23), LBM_RF(24),
+            LBM_RF(25), LBM_RF(26));
+}
+#undef LBM_RF
+
 template <class S, bool BIG> struct Field {
     using T = typename S::T;
     using R = typename S::R;
@@ -131,6 +156,10 @@ template <class S, bool BIG> struct Field {
     __device__ __forceinline__ R ld(uint32_t lane, uint32_t uni) const {
         if (BIG) return S::ld(ptr + ((size_t)lane + uni));
         return Bits<S>::get(rsrc, lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T));
+    }
+    // Float16C, 32-bit addressing: the sign-extended code (decoded by f16c_decode_sext).
+    __device__ __forceinline__ int32_t ld_code(uint32_t lane, uint32_t uni) const {
+        return Bits<S>::raw(rsrc, lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T));
     }
     __device__ __forceinline__ void st(uint32_t lane, uint32_t uni, R v) const {
         if (BIG) S::st(ptr + ((size_t)lane + uni), v);
@@ -148,8 +177,10 @@ template <class L> struct TileCtx {
     static constexpr int MAXR = 8;  // halo rounds per lane (hcells <= 8 * tile_cells, checked on the host)
     int x, y, z;          // node
     int tid;
+    uint32_t loc;         // node index inside its tile
+    int wz0, wtz;         // z offset inside the tile of the workgroup's slab and its layer count
     int hx, hxy;          // LDS halo strides
-    int center;           // own position in the LDS tile
+    int center;           // own position in the LDS tile (the workgroup's slab + halo)
     uint32_t tile;        // tile id
     int bx, by, bz;       // tile coordinates (uniform)
     uint32_t fbase;       // element index of (x, q = 0) in f
@@ -159,12 +190,18 @@ template <class L> struct TileCtx {
     uint8_t own;
     uint32_t hv[MAXR];    // fetched halo flags
 
-    __device__ __forceinline__ void init_pos(const Geo& g, uint32_t tile_id) {
+    // The workgroup covers z layers [part 2^lzb, (part + 1) 2^lzb) of the tile (part 0 and
+    // lzb = tzb: the whole tile); lane tid is node tid + part 2^lzb tx ty of that tile.
+    __device__ __forceinline__ void init_pos(const Geo& g, uint32_t tile_id, uint32_t part, int lzb) {
         constexpr int D = L::D, Q = L::Q;
         tile = (uint32_t)__builtin_amdgcn_readfirstlane(tile_id);
         tid = threadIdx.x;
         const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
-        const int lx = tid & (tx - 1), ly = (tid >> g.txb) & (ty - 1), lz = tid >> (g.txb + g.tyb);
+        wz0 = (int)part << lzb;
+        wtz = 1 << lzb;
+        const int wlz = tid >> (g.txb + g.tyb);  // layer inside the slab
+        const int lx = tid & (tx - 1), ly = (tid >> g.txb) & (ty - 1), lz = wz0 + wlz;
+        loc = (uint32_t)tid + ((uint32_t)wz0 << (g.txb + g.tyb));
         bx = __builtin_amdgcn_readfirstlane((int)(tile % (uint32_t)g.Tx));
         const int byz = (int)(tile / (uint32_t)g.Tx);
         by = __builtin_amdgcn_readfirstlane(byz % g.Ty);
@@ -174,10 +211,10 @@ template <class L> struct TileCtx {
         z = (bz << g.tzb) + lz;
         hx = tx + 2;
         hxy = hx * (ty + 2);
-        center = (lz + (D == 3 ? 1 : 0)) * hxy + (ly + 1) * hx + (lx + 1);
+        center = (wlz + (D == 3 ? 1 : 0)) * hxy + (ly + 1) * hx + (lx + 1);
         tc = (uint32_t)g.tile_cells;
         const uint32_t TQ = tc * (uint32_t)Q;
-        fbase = tile * TQ + (uint32_t)tid;
+        fbase = tile * TQ + loc;
 
         const int Ts[3] = {1, g.Tx, g.Tx * g.Ty};
         const int ls[3] = {1, tx, tx * ty};
@@ -229,11 +266,10 @@ template <class L> struct TileCtx {
         return field_index(g, xn, yn, zn, K, k);
     }
 
-    // Loads the flags of the tile and its halo into registers (no LDS traffic yet).
+    // Loads the flags of the workgroup's slab and its halo into registers (no LDS traffic yet).
     __device__ __forceinline__ void fetch_flags(const Geo& g, const uint8_t* __restrict__ flags) {
         constexpr int D = L::D;
-        const int tz = 1 << g.tzb;
-        const int hcells = hxy * (D == 3 ? tz + 2 : 1);
+        const int hcells = hxy * (D == 3 ? wtz + 2 : 1);
         const int nthr = blockDim.x;
 #pragma unroll
         for (int r = 0; r < MAXR; ++r) {
@@ -253,7 +289,7 @@ template <class L> struct TileCtx {
                 gy -= gy >= g.ny ? g.ny : 0;
                 int gz = 0;
                 if (D == 3) {
-                    gz = (bz << g.tzb) + iz - 1;
+                    gz = (bz << g.tzb) + wz0 + iz - 1;
                     gz += gz < 0 ? g.nz : 0;
                     gz -= gz >= g.nz ? g.nz : 0;
                 }
@@ -265,8 +301,7 @@ template <class L> struct TileCtx {
     // Writes the fetched flags to LDS and synchronises the workgroup.
     __device__ __forceinline__ void commit_flags(const Geo& g, uint8_t* sfl) {
         constexpr int D = L::D;
-        const int tz = 1 << g.tzb;
-        const int hcells = hxy * (D == 3 ? tz + 2 : 1);
+        const int hcells = hxy * (D == 3 ? wtz + 2 : 1);
         const int nthr = blockDim.x;
         uint32_t any = 0;
 #pragma unroll
@@ -328,26 +363,41 @@ __device__ __forceinline__ uint32_t boundary_tile(const Geo& g, const uint32_t* 
 // ops/load_and_store.mojo:164-221) or the esoteric slot loads
 // (esoteric_pull_load_f_vec, ops/load_and_store.mojo:11-86). Every address is
 // in bounds (periodic wrap), whatever the flags say.
-template <class L, class S, int MODE, bool BIG>
-__device__ __forceinline__ void load_populations(const TileCtx<L>& t, const Field<S, BIG>& f,
-                                                 typename S::R (&fv)[L::Q]) {
+template <class L, int MODE, class LD>
+__device__ __forceinline__ void for_each_pull(const TileCtx<L>& t, LD&& ld) {
     constexpr int Q = L::Q;
     if (MODE == MODE_DB) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fv[q] = f.ld(t.nb(q, -1), (uint32_t)q * t.tc);
+        for (int q = 0; q < Q; ++q) ld(q, t.nb(q, -1), (uint32_t)q * t.tc);
     } else {
-        fv[0] = f.ld(t.fbase, 0);
+        ld(0, t.fbase, 0u);
 #pragma unroll
         for (int p = 1; p < Q; p += 2) {
             const int n = p + 1;
             if (MODE == MODE_EP_EVEN) {
-                fv[p] = f.ld(t.fbase, (uint32_t)p * t.tc);
-                fv[n] = f.ld(t.nb(p, +1), (uint32_t)n * t.tc);
+                ld(p, t.fbase, (uint32_t)p * t.tc);
+                ld(n, t.nb(p, +1), (uint32_t)n * t.tc);
             } else {
-                fv[p] = f.ld(t.fbase, (uint32_t)n * t.tc);
-                fv[n] = f.ld(t.nb(p, +1), (uint32_t)p * t.tc);
+                ld(p, t.fbase, (uint32_t)n * t.tc);
+                ld(n, t.nb(p, +1), (uint32_t)p * t.tc);
             }
         }
+    }
+}
+
+template <class L, class S, int MODE, bool BIG>
+__device__ __forceinline__ void load_populations(const TileCtx<L>& t, const Field<S, BIG>& f,
+                                                 typename S::R (&fv)[L::Q]) {
+    constexpr int Q = L::Q;
+    if constexpr (S::id == LBM_STORE_F16C && !BIG) {
+        // all codes loaded sign-extended, then decoded: 3 instructions per population
+        int32_t code[Q];
+        for_each_pull<L, MODE>(t, [&](int q, uint32_t lane, uint32_t uni) { code[q] = f.ld_code(lane, uni); });
+        register_fence<Q>(code);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fv[q] = f16c_decode_sext(code[q]);
+    } else {
+        for_each_pull<L, MODE>(t, [&](int q, uint32_t lane, uint32_t uni) { fv[q] = f.ld(lane, uni); });
     }
 }
 
@@ -579,11 +629,11 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
 // plain kernels, whose register budget it would otherwise set.
 template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS>
 __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, uint32_t tile, uint32_t force_slot,
-                                          uint8_t* sfl) {
+                                          uint8_t* sfl, uint32_t part, int lzb) {
     using R = typename S::R;
     constexpr int Q = L::Q;
     TileCtx<L> t;
-    t.init_pos(p.g, tile);
+    t.init_pos(p.g, tile, part, lzb);
     const uint32_t kind = p.summary ? boundary_tile<L::D>(p.g, p.summary, t.bx, t.by, t.bz) : 3u;
     const bool boundary = (kind & 1u) != 0;
     const int moving = p.moving && (kind & 2u);
@@ -607,7 +657,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
                 uint32_t rebuilt = 0;
                 if (links && (t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY)
                     rebuilt = bouzidi_links<L, S, DDF, BIG>(t, sfl, fin, p.link_nodes, p.link_q, p.n_links,
-                                                            t.tile * t.tc + (uint32_t)t.tid, fv, force);
+                                                            t.tile * t.tc + t.loc, fv, force);
                 apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, fin, p.bc, moving, p.eqbc, fv, rebuilt);
             }
         }
@@ -632,11 +682,14 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
     }
 }
 
+// One workgroup per z slab of 2^wzb layers: 2^(tzb - wzb) workgroups per tile, numbered
+// tile-major, so the force records keep the whole-tile order (slot = unit, wave inside it).
 template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS>
 __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParams<typename S::R> p) {
     extern __shared__ uint8_t sfl[];
-    const uint32_t tile = (uint32_t)tile_of_block(p.g);
-    step_tile<L, S, MODE, COLL, LES, DDF, BIG, LINKS>(p, tile, tile, sfl);
+    const int s = p.g.tzb - p.g.wzb;
+    const uint32_t unit = (uint32_t)tile_of_block(p.g, s);
+    step_tile<L, S, MODE, COLL, LES, DDF, BIG, LINKS>(p, unit >> s, unit, sfl, unit & ((1u << s) - 1u), p.g.wzb);
 }
 
 // The step with the learned closure, held to at most 128 VGPRs (two 512-lane workgroups per CU)
@@ -646,7 +699,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
     step_kernel_mlp(StepParams<typename S::R> p) {
     extern __shared__ uint8_t sfl[];
     const uint32_t tile = (uint32_t)tile_of_block(p.g);
-    step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, tile, tile, sfl);
+    step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, tile, tile, sfl, 0, p.g.tzb);
 }
 
 // get_velocity_gradient (src/lbm/kernels/utils/finite_difference.mojo:15-136), dx = 1: the
@@ -767,7 +820,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) __attribute__((amdgpu
     extern __shared__ uint8_t sfl[];
     __shared__ double red[D + 1][16];
     TileCtx<L> t;
-    t.init_pos(p.g, macro_tile(p.g));
+    t.init_pos(p.g, macro_tile(p.g), 0, p.g.tzb);
     const Field<S, BIG> f(p.f, p.f_bytes);
     t.fetch_flags(p.g, p.flags);
     R fv[Q];
@@ -780,7 +833,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) __attribute__((amdgpu
             if (p.link_nodes && (t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY) {
                 double unused[3] = {0, 0, 0};
                 rebuilt = bouzidi_links<L, S, DDF, BIG>(t, sfl, f, p.link_nodes, p.link_q, p.n_links,
-                                                        t.tile * t.tc + (uint32_t)t.tid, fv, unused);
+                                                        t.tile * t.tc + t.loc, fv, unused);
             }
         }
         apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, f, p.bc, p.moving, p.eqbc, fv, rebuilt);

@@ -13,12 +13,14 @@ template <class L, class S> struct LaunchImpl {
 
     template <int MODE, int COLL, int LES, bool DDF, bool LINKS>
     static int step_l(const Handle& h, const StepParams<R>& p, hipStream_t st) {
+        const int s = h.g.tzb - h.g.wzb;  // 2^s workgroups per tile, one z slab each
+        const dim3 grid((unsigned)h.g.n_tiles << s), block((unsigned)h.g.tile_cells >> s);
         if (h.big)
-            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, true, LINKS>), dim3(h.g.n_tiles),
-                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
+            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, true, LINKS>), grid, block, h.lds_bytes, st,
+                               p);
         else
-            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, false, LINKS>), dim3(h.g.n_tiles),
-                               dim3(h.g.tile_cells), h.lds_bytes, st, p);
+            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, false, LINKS>), grid, block, h.lds_bytes, st,
+                               p);
         return check_launch("step_kernel");
     }
 

@@ -43,3 +43,33 @@ def test_encode_matches_oracle(gpu):
                         np.nextafter(mids, np.float32(3)), rand, specials])
     got = _convert(gpu, x, True)
     np.testing.assert_array_equal(got, O.f16c_encode(x))
+
+
+def test_encode_all_inputs(gpu):
+    """Every one of the 2^32 fp32 bit patterns (denormals, infinities and NaNs included),
+    encoded on the device, equals the oracle's code: in chunks of 2^27, the oracle run on
+    host threads."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+
+    chunk, parts = 1 << 27, 16
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = torch.empty(chunk, dtype=torch.uint16, device=gpu)
+    ref = np.empty(chunk, np.uint16)
+
+    def oracle_part(x, k):
+        n = chunk // parts
+        ref[k * n:(k + 1) * n] = O.f16c_encode(x[k * n:(k + 1) * n])
+
+    with ThreadPoolExecutor(parts) as pool:
+        for c in range((1 << 32) // chunk):
+            start = c * chunk
+            src = torch.arange(start, start + chunk, dtype=torch.int64, device=gpu).to(torch.int32)
+            _lib.check(_lib.lib().lbm_f16c_convert(src.data_ptr(), out.data_ptr(), chunk, 1, st))
+            x = np.arange(start, start + chunk, dtype=np.uint64).astype(np.uint32).view(np.float32)
+            list(pool.map(lambda k: oracle_part(x, k), range(parts)))
+            got = out.cpu().numpy()
+            bad = np.flatnonzero(got != ref)
+            assert bad.size == 0, (f"chunk {c}: {bad.size} codes differ, first input bits "
+                                   f"{int(start + bad[0]):#010x}: {got[bad[0]]:#06x} vs {ref[bad[0]]:#06x}")
+            del src

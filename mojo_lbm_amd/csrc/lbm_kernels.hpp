@@ -698,8 +698,9 @@ template <class L, class S, int MODE, int COLL, bool DDF>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8)))
     step_kernel_mlp(StepParams<typename S::R> p) {
     extern __shared__ uint8_t sfl[];
-    const uint32_t tile = (uint32_t)tile_of_block(p.g);
-    step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, tile, tile, sfl, 0, p.g.tzb);
+    const int s = p.g.tzb - p.g.wzb;
+    const uint32_t unit = (uint32_t)tile_of_block(p.g, s);
+    step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, unit >> s, unit, sfl, unit & ((1u << s) - 1u), p.g.wzb);
 }
 
 // get_velocity_gradient (src/lbm/kernels/utils/finite_difference.mojo:15-136), dx = 1: the

@@ -40,8 +40,9 @@ template <class L, class S> struct LaunchImpl {
             if (!p.mlp) return set_error(LBM_ERR_INVALID, "the learned closure needs lbm_set_mlp first");
             if (h.g.tile_cells % 64)
                 return set_error(LBM_ERR_UNSUPPORTED, "the learned closure needs tiles of whole 64-lane waves");
-            hipLaunchKernelGGL((step_kernel_mlp<L, S, MODE, COLL, DDF>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
-                               h.lds_bytes, st, p);
+            const int s = h.g.tzb - h.g.wzb;  // 2^s workgroups per tile, one z slab each
+            hipLaunchKernelGGL((step_kernel_mlp<L, S, MODE, COLL, DDF>), dim3((unsigned)h.g.n_tiles << s),
+                               dim3((unsigned)h.g.tile_cells >> s), h.lds_bytes, st, p);
             return check_launch("step_kernel");
         } else {
             return set_error(LBM_ERR_UNSUPPORTED, "the learned closure needs a 3-D lattice");

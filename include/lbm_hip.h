@@ -302,9 +302,10 @@ int lbm_init_f_from_fields(lbm_handle* h, void* f, double rho, const void* u, co
  *                      two >= 64 that is a whole number of the tile's z layers, so
  *                      2^k workgroups share a tile, each with a z slab of it (the
  *                      (x,y,z,q) layout is unchanged); 0 or tile_cells = one
- *                      workgroup per tile. Default: 256 for Float16C storage on
- *                      3-D tiles of more than 256 nodes, else one workgroup per
- *                      tile. Results are identical for every value.
+ *                      workgroup per tile. Default: 256 for Float16C storage or
+ *                      the learned closure on 3-D tiles of more than 256 nodes,
+ *                      else one workgroup per tile. Results are identical for
+ *                      every value.
  * Returns LBM_ERR_INVALID for an unknown option or value.
  */
 enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2, LBM_OPT_ADDR64 = 3, LBM_OPT_STEP_LANES = 4 };

@@ -95,11 +95,12 @@ Geo make_geo(const lbm_desc* d) {
     // XCD-contiguous tile order when the tile count allows it (measured faster on MI355X for
     // the wall-bounded and Float16C cases); lbm_set_option(LBM_OPT_XCD_REMAP) overrides it.
     g.xcd_chunk = (g.n_tiles % 8 == 0) ? g.n_tiles / 8 : 0;
-    // One step workgroup per tile, except Float16C 3-D tiles of more than 256 nodes: 256-lane
-    // workgroups on z slabs (measured 9-13 % faster on MI355X for 8x8x8 tiles, where the
-    // codec makes the step instruction-bound; FP32 does not gain). LBM_OPT_STEP_LANES overrides.
+    // One step workgroup per tile, except 3-D tiles of more than 256 nodes with Float16C storage
+    // or the learned closure: 256-lane workgroups on z slabs (measured on MI355X with 8x8x8
+    // tiles: Float16C 9-13 % faster at 256^3, the learned closure 7 % at 128^3; FP32 with
+    // SRT does not gain at 256^3). LBM_OPT_STEP_LANES overrides.
     g.wzb = g.tzb;
-    if (d->storage == LBM_STORE_F16C && d->D == 3 && g.tile_cells > 256) {
+    if ((d->storage == LBM_STORE_F16C || d->les == LBM_LES_MLP) && d->D == 3 && g.tile_cells > 256) {
         const int wzb = step_slab_bits(g, d->D, 256);
         if (wzb >= 0) g.wzb = wzb;
     }

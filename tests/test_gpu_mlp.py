@@ -207,3 +207,23 @@ def test_closure_errors(gpu):
     bad[5] = np.nan
     with pytest.raises(_lib.LbmError):
         _lib.check(run.L.lbm_set_mlp(run.h.ptr, bad.ctypes.data, len(bad)))
+
+
+@pytest.mark.parametrize("storage,method,lanes", [(0, 0, 256), (0, 1, 128), (1, 0, 256)])
+def test_closure_step_lanes_bit_identical(gpu, storage, method, lanes):
+    """The learned-closure step with z-slab workgroups (LBM_OPT_STEP_LANES) against one
+    workgroup per tile: the same device arithmetic per node, so bit-identical populations."""
+    desc, f, bc, flags, P = _closure_case(storage, method, seed=40 + storage + 3 * method)
+    runs = [U.DeviceRun(desc, f, bc, flags) for _ in range(2)]
+    for r in runs:
+        _lib.check(r.L.lbm_set_mlp(r.h.ptr, P.ctypes.data, len(P)))
+    whole = desc.tx * desc.ty * desc.tz
+    _lib.check(runs[0].L.lbm_set_option(runs[0].h.ptr, _lib.OPT_STEP_LANES, whole))
+    _lib.check(runs[1].L.lbm_set_option(runs[1].h.ptr, _lib.OPT_STEP_LANES, lanes))
+    for k in range(3):
+        for r in runs:
+            if method == 0:
+                r.step_db(0.51)
+            else:
+                r.step_ep(0.51, k % 2 == 0)
+    assert np.array_equal(runs[0].f_host().view(np.uint8), runs[1].f_host().view(np.uint8))

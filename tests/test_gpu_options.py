@@ -76,17 +76,20 @@ def test_addr64_kernels_match_oracle(gpu, lat, storage, ddf, method):
     assert np.allclose(tot, O.reduce_moments(desc, a, bc, flags, after_odd=True), rtol=1e-12, atol=1e-12)
 
 
-STEP_LANES_CASES = [("D3Q19", 1, 1, 0, (8, 8, 8), 256), ("D3Q19", 1, 1, 1, (8, 8, 8), 256),
-                    ("D3Q19", 0, 0, 0, (8, 8, 8), 64), ("D3Q19", 0, 0, 1, (8, 4, 8), 128),
-                    ("D3Q27", 2, 0, 0, (8, 8, 8), 128), ("D3Q27", 1, 1, 1, (16, 8, 4), 256)]
+STEP_LANES_CASES = [("D3Q19", 1, 1, 0, (8, 8, 8), 256, (16, 16, 16)), ("D3Q19", 1, 1, 1, (8, 8, 8), 256, (16, 16, 16)),
+                    ("D3Q19", 0, 0, 0, (8, 8, 8), 64, (16, 16, 16)), ("D3Q19", 0, 0, 1, (8, 4, 8), 128, (16, 16, 16)),
+                    ("D3Q27", 2, 0, 0, (8, 8, 8), 128, (16, 16, 16)), ("D3Q27", 1, 1, 1, (16, 8, 4), 256, (16, 16, 16)),
+                    # 12 tiles: no XCD remap (the tile count is not a multiple of 8)
+                    ("D3Q19", 1, 1, 0, (8, 8, 8), 256, (24, 16, 16)), ("D3Q19", 0, 0, 1, (8, 8, 8), 128, (24, 16, 16))]
 
 
-@pytest.mark.parametrize("lat,storage,ddf,method,tile,lanes", STEP_LANES_CASES)
-def test_step_lanes_match_oracle(gpu, lat, storage, ddf, method, tile, lanes):
+@pytest.mark.parametrize("lat,storage,ddf,method,tile,lanes,grid", STEP_LANES_CASES)
+def test_step_lanes_match_oracle(gpu, lat, storage, ddf, method, tile, lanes, grid):
     """Several workgroups per tile (one z slab each, LBM_OPT_STEP_LANES): populations bit-identical
     to the oracle after every step, with walls, a moving lid and Smagorinsky LES."""
-    print(f"case step lanes {lat} storage={storage} ddf={ddf} method={method} tile={tile} lanes={lanes}", flush=True)
-    desc = U.make_desc(lat, (16, 16, 16), tile=tile, storage=storage, ddf=ddf, method=method, les=1)
+    print(f"case step lanes {lat} storage={storage} ddf={ddf} method={method} tile={tile} lanes={lanes} "
+          f"grid={grid}", flush=True)
+    desc = U.make_desc(lat, grid, tile=tile, storage=storage, ddf=ddf, method=method, les=1)
     f, bc, flags = U.perturbed_state(desc, seed=5 + storage)
     run = U.DeviceRun(desc, f, bc, flags)
     _lib.check(run.L.lbm_set_option(run.h.ptr, _lib.OPT_STEP_LANES, lanes))

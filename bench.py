@@ -421,6 +421,8 @@ def main():
                     help="run one variant: fp32 | f16c | f16c_ep | fp32_ep | fp32_periodic | f16c_periodic | cylinders | les "
                          "| outputs")
     ap.add_argument("--xcd-remap", type=int, default=None, help="override LBM_OPT_XCD_REMAP (0 or 1)")
+    ap.add_argument("--step-lanes", type=int, default=None,
+                    help="override LBM_OPT_STEP_LANES (lanes per step workgroup; 0 = one workgroup per tile)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only a plain 1 GiB -> 1 GiB device copy kernel (for scaling rocprofv3 byte counters)")
     ap.add_argument("--accuracy", default=None,
@@ -501,6 +503,8 @@ def main():
         _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_STATIC_FLAGS, 1))
         if args.xcd_remap is not None:
             _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_XCD_REMAP, args.xcd_remap))
+        if args.step_lanes is not None:
+            _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_STEP_LANES, args.step_lanes))
         steps = args.steps if (name == "fp32" or args.only) else max(100, args.steps // 2)
         t = time_run(run, tau, steps, args.warmup)
         B = algorithmic_bytes(19, N, N, N, s, walls)

@@ -542,8 +542,9 @@ def main():
                         f"u_lid=0.1, Re=100, tau={tau:.3f}",
             "lattice": "D3Q19", "grid": [N, N, N], "tile": [8, 8, 8], "method": "double_buffer",
             "storage": head, "parallelism": "single GPU", "static_flags": True,
-            "step_workgroups": "FP32: one 512-lane workgroup per tile; Float16C: two 256-lane z slabs per tile "
-                               "(the library default, LBM_OPT_STEP_LANES)",
+            "step_workgroups": ("FP32: one 512-lane workgroup per tile; Float16C: two 256-lane z slabs per tile "
+                                "(the library default, LBM_OPT_STEP_LANES)") if args.step_lanes is None
+                               else f"LBM_OPT_STEP_LANES = {args.step_lanes} for every variant",
         },
         "roofline": {
             "bound": "hbm",

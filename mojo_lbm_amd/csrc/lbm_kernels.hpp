@@ -799,14 +799,10 @@ __device__ __forceinline__ void store_rowmajor(const Geo& g, int bx, int by, int
     }
 }
 
-// Tile of a macro_kernel workgroup: the launch order (XCD-contiguous) walks tiles with z
-// fastest, so the workgroups that fill one row-major cache line of output run together.
-__device__ __forceinline__ uint32_t macro_tile(const Geo& g) {
-    const uint32_t t = (uint32_t)tile_of_block(g);
-    const uint32_t bz = t % (uint32_t)g.Tz, rest = t / (uint32_t)g.Tz;
-    const uint32_t by = rest % (uint32_t)g.Ty, bx = rest / (uint32_t)g.Ty;
-    return bx + (uint32_t)g.Tx * (by + (uint32_t)g.Ty * bz);
-}
+// Tile of a macro_kernel workgroup: the step's order (x fastest, XCD-contiguous), so the
+// pulls share their halo lines in L2 as the step's do. (Walking tiles z-fastest, so that the
+// workgroups filling one row-major output line run together, measured 4-6 % slower at 256^3.)
+__device__ __forceinline__ uint32_t macro_tile(const Geo& g) { return (uint32_t)tile_of_block(g); }
 
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125), the FP64 moment
 // reduction and calculate_Q_criterion (src/lbm/output/Q_criterion.mojo:90-215): re-stream +

@@ -584,9 +584,10 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
             const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
             const uint32_t l0 = (i << 4) & (uint32_t)(g.tile_cells - 1);
             const uint32_t tile = (i << 4) / (uint32_t)g.tile_cells;
+            uint32_t regs[16], solid = 0;
+#pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const uint32_t fb = (w[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
-                if (fb == 0u) continue;
                 const int l = (int)l0 + j;
                 const int lx = l & (tx - 1), ly = (l >> g.txb) & (ty - 1), lz = l >> (g.txb + g.tyb);
                 const uint32_t mx = axis_regions(0, 0) | (lx == 0 ? axis_regions(0, -1) : 0u) |
@@ -596,16 +597,25 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
                 const uint32_t mz = D == 2 ? axis_regions(2, 0)
                                            : (axis_regions(2, 0) | (lz == 0 ? axis_regions(2, -1) : 0u) |
                                               (lz == tz - 1 ? axis_regions(2, 1) : 0u));
-                const uint32_t regs = mx & my & mz;
-                mask |= regs;
-                if (moving && (fb & 0x0Fu) == LBM_FLAG_SOLID) {
-                    const uint32_t tc = (uint32_t)g.tile_cells;
-                    bool mv = false;
+                regs[j] = fb != 0u ? (mx & my & mz) : 0u;
+                mask |= regs[j];
+                solid |= ((fb & 0x0Fu) == LBM_FLAG_SOLID ? 1u : 0u) << j;
+            }
+            if (moving && solid) {
+                // the bc of all 16 nodes, loaded together (independent loads, one wait), then
+                // tested for the SOLID ones
+                const uint32_t tc = (uint32_t)g.tile_cells;
+                R b[D + 1][16];
 #pragma unroll
-                    for (int d = 0; d < D; ++d) mv = mv || bc[(tile * (D + 1) + d) * tc + (uint32_t)l] != (R)0;
-                    const R rw = bc[(tile * (D + 1) + D) * tc + (uint32_t)l];
-                    mv = mv || !isfinite(rw);
-                    if (mv) mmask |= regs;
+                for (int d = 0; d <= D; ++d)
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) b[d][j] = bc[(tile * (D + 1) + d) * tc + l0 + (uint32_t)j];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    bool mv = !isfinite(b[D][j]);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) mv = mv || b[d][j] != (R)0;
+                    if (((solid >> j) & 1u) && mv) mmask |= regs[j];
                 }
             }
         }

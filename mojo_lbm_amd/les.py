@@ -40,6 +40,7 @@ U_LID, RE, CS, FACTOR = 0.05, 10000.0, 0.1, 4
 N_FINE, N_COARSE, FINE_LO = 512, 128, 6  # fine cavity walls at 6 and 511
 OFFSET = FINE_LO - FACTOR + 1  # coarse I covers fine FACTOR*I + OFFSET + (0..3) = 4I+3 .. 4I+6
 MLP_NPARAM, MLP_TOTAL = 1409, 1446
+MLP_NUSCALE = MLP_TOTAL - 1  # output scale of nu_t in the parameter block
 
 
 def set_sizes(n_coarse: int, re: float = 10000.0) -> None:
@@ -288,13 +289,21 @@ def _coarse_u(c: Cavity):
     return u.view(N_COARSE, N_COARSE, N_COARSE, 3)[1:-1, 1:-1, 1:-1].double()
 
 
+def scaled_params(params, factor):
+    """The closure with its nu_t output scaled by factor (an a-posteriori calibration)."""
+    p = np.array(params, np.float32, copy=True)
+    p[MLP_NUSCALE] = np.float32(p[MLP_NUSCALE] * factor)
+    return p
+
+
 def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=print, stats_steps=0,
-             sample_every=25):
+             sample_every=25, nu_scales=()):
     """A-posteriori comparison of coarse closures against the filtered held-out fine run: velocity
     rel-L2 at checkpoints, then (stats_steps > 0) the rel-L2 of the time-mean velocity and of the
     rms velocity fluctuation over stats_steps further coarse steps sampled every sample_every steps
     (instantaneous fields decorrelate from the fine run within about a thousand coarse steps; the
-    statistics do not)."""
+    statistics do not). Each factor of nu_scales adds a coarse run of the closure with its nu_t
+    scaled by that factor ("learned_x<factor>")."""
     torch = _torch()
     t0 = time.time()
     fine = Cavity(N_FINE, FINE_LO, tau_for(N_FINE), storage=_lib.STORE_F16C, collision=collision, seed=seed)
@@ -310,6 +319,8 @@ def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=
     rho0 = ref0[..., 0].contiguous().view(-1)
     u0 = ref0[..., 1:4].contiguous().view(-1)
     models = {"none": (0, None), "smagorinsky": (1, None), "learned": (2, params)}
+    for fct in nu_scales:
+        models[f"learned_x{fct:g}"] = (2, scaled_params(params, fct))
     coarse = {}
     for name, (les, p) in models.items():
         c = Cavity(N_COARSE, 0, tau_for(N_COARSE), storage=_lib.STORE_F32, les=les, seed=seed, mlp=p)
@@ -406,9 +417,16 @@ def main(argv=None):
     e.add_argument("--coarse", type=int, default=128)
     e.add_argument("--stats-steps", type=int, default=0, help="coarse steps of time statistics after the checkpoints")
     e.add_argument("--sample-every", type=int, default=25)
+    e.add_argument("--nu-scales", default="", help="comma-separated factors: extra closure runs with nu_t scaled")
     e.add_argument("--out", required=True)
+    sc = sub.add_parser("scale", help="write a copy of a parameter file with the nu_t output scaled")
+    sc.add_argument("--params", required=True)
+    sc.add_argument("--factor", type=float, required=True)
+    sc.add_argument("--note", default="")
+    sc.add_argument("--out", required=True)
     a = ap.parse_args(argv)
-    set_sizes(a.coarse)
+    if a.cmd != "scale":
+        set_sizes(a.coarse)
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
@@ -433,12 +451,19 @@ def main(argv=None):
         with open(a.out, "w") as fh:
             json.dump(out, fh)
         log(json.dumps({k: v for k, v in skill.items() if k != "val_loss_history"}))
+    elif a.cmd == "scale":
+        saved = json.load(open(a.params))
+        saved["params"] = [float(v) for v in scaled_params(saved["params"], a.factor)]
+        saved.setdefault("calibration", []).append({"nu_scale_factor": a.factor, "note": a.note})
+        with open(a.out, "w") as fh:
+            json.dump(saved, fh)
     else:
         saved = json.load(open(a.params))
         P = np.asarray(saved["params"], np.float32)
         coll = saved.get("config", {}).get("collision", a.collision)
+        scales = [float(v) for v in a.nu_scales.split(",") if v.strip()]
         res = evaluate(P, a.seed, a.spinup, a.horizon, collision=coll, log=log, stats_steps=a.stats_steps,
-                       sample_every=a.sample_every)
+                       sample_every=a.sample_every, nu_scales=scales)
         res["fine_collision"] = coll
         with open(a.out, "w") as fh:
             json.dump(res, fh)

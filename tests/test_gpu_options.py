@@ -80,7 +80,9 @@ STEP_LANES_CASES = [("D3Q19", 1, 1, 0, (8, 8, 8), 256, (16, 16, 16)), ("D3Q19", 
                     ("D3Q19", 0, 0, 0, (8, 8, 8), 64, (16, 16, 16)), ("D3Q19", 0, 0, 1, (8, 4, 8), 128, (16, 16, 16)),
                     ("D3Q27", 2, 0, 0, (8, 8, 8), 128, (16, 16, 16)), ("D3Q27", 1, 1, 1, (16, 8, 4), 256, (16, 16, 16)),
                     # 12 tiles: no XCD remap (the tile count is not a multiple of 8)
-                    ("D3Q19", 1, 1, 0, (8, 8, 8), 256, (24, 16, 16)), ("D3Q19", 0, 0, 1, (8, 8, 8), 128, (24, 16, 16))]
+                    ("D3Q19", 1, 1, 0, (8, 8, 8), 256, (24, 16, 16)), ("D3Q19", 0, 0, 1, (8, 8, 8), 128, (24, 16, 16)),
+                    # one-layer slabs of a 16x16x2 tile (the Float16C default for this tile shape)
+                    ("D3Q19", 1, 1, 1, (16, 16, 2), 256, (32, 16, 16))]
 
 
 @pytest.mark.parametrize("lat,storage,ddf,method,tile,lanes,grid", STEP_LANES_CASES)

@@ -26,7 +26,11 @@
  *   esoteric pull ......... bitwise equality with double buffer in periodic boxes; slot ownership
  *   Smagorinsky ........... closed form on a synthetic shear f_neq
  *   KBC ................... gamma = 2 reduces to SRT; conservation; D2Q9 shear table
- *   equilibrium BC ........ closed form (node reset to f_eq of prescribed rho, u)
+ *   equilibrium BC ........ node reset to f_eq of prescribed rho, u; with free (NaN) u and/or rho the
+ *                           local moments rebuilt node by node with out-of-grid pulls at the rest value
+ *                           (w_q, 0 with DDF), on both x faces incl. edges and corners
+ *   FP64 totals ........... uniform states give N rho0 and N rho0 u0; only non-SOLID, non-EXCLUDE nodes
+ *                           count; the lid's momentum injection closed form; equal to fsum of the fields
  *   initialize_f_from_func  moments of f_eq(rho, u(x)); Pi_neq = -2 rho cs^2 (tau - 1) S of the gradient
  *                           term; esoteric store = the double-buffer fill moved by the odd pattern
  *   Q-criterion ........... solid-body rotation (Q = |Omega|^2), uniform strain (Q = -9 eps^2 / (4 tau^2)),

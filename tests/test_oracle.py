@@ -346,6 +346,128 @@ def test_equilibrium_bc_resets_node():
         np.testing.assert_allclose(got, feq, rtol=0, atol=1e-15)
 
 
+def _feq_np(c, w, rho, u):
+    cu = c @ u
+    return w * rho * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * u @ u)
+
+
+@pytest.mark.parametrize("ddf", [0, 1])
+@pytest.mark.parametrize("free", ["u", "rho", "both"])
+@pytest.mark.parametrize("side", ["-X", "+X"])
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19"])
+def test_equilibrium_bc_free_moments(lat, side, free, ddf):
+    """EQUILIBRIUM column with a free (NaN) velocity and/or density: the node is reset to
+    f_eq(rho*, u*) where (rho*, u*) are the moments of the pulled populations, each pull that
+    leaves the grid on any axis replaced by the rest value w_q (0 with DDF) and rho* += 1 with
+    DDF (boundary_condition.mojo:87-204). Expected values are built here node by node from the
+    stored field. The column on the opposite face holds a different state, so a pull that wraps
+    round instead of taking the rest value, a rest value of the wrong kind, a pull in the wrong
+    direction or a swapped NaN selection all change the result by O(1e-2)."""
+    n = (16, 16, 1) if lat == "D2Q9" else (8, 8, 8)
+    desc = U.make_desc(lat, n, storage=2, eq=True, ddf=ddf)
+    D, Q = desc.D, desc.Q
+    c, w, _ = O.lattice(D, Q)
+    cc = c[:, :D].astype(np.float64)
+    rho0, u0 = 1.03, np.array([0.04, -0.02, 0.01][:D])
+    rho1, u1 = 0.97, np.array([-0.03, 0.05, -0.02][:D])
+    shift = w if ddf else 0.0
+    fv0 = _feq_np(cc, w, rho0, u0) - shift
+    fv1 = _feq_np(cc, w, rho1, u1) - shift
+    f, bc, flags = O.alloc(desc)
+    xo = desc.nx - 1 if side == "-X" else 0  # the column a wrapped pull would reach
+    for z in range(n[2] if D == 3 else 1):
+        for y in range(n[1]):
+            for x in range(n[0]):
+                fv = fv1 if x == xo else fv0
+                for q in range(Q):
+                    f[O.addr(desc, x, y, z, q, Q)] = fv[q]
+    u_bc = np.array([0.02, 0.01, -0.015][:D])
+    rho_bc = 1.05
+    O.set_exterior_wall(desc, flags, bc, side, 2, None if free in ("u", "both") else list(u_bc),
+                        float("nan") if free in ("rho", "both") else rho_bc)
+    out = f.copy()
+    O.step_db(desc, out, f, bc, flags, 0.7)
+    xe = 0 if side == "-X" else desc.nx - 1
+    rest = np.zeros(Q) if ddf else w
+    nn = [desc.nx, desc.ny, desc.nz]
+    n_oob_seen = set()
+    for z in range(n[2] if D == 3 else 1):
+        for y in range(n[1]):
+            idx = (xe, y, z)
+            fq = np.empty(Q)
+            for q in range(Q):
+                p = [idx[i] - (c[q][i] if i < D else 0) for i in range(3)]
+                oob = any(p[i] < 0 or p[i] >= nn[i] for i in range(D))
+                if oob:
+                    fq[q] = rest[q]
+                else:
+                    fq[q] = (fv1 if p[0] == xo else fv0)[q]
+            n_oob_seen.add(int(sum(any((idx[i] - c[q][i]) < 0 or (idx[i] - c[q][i]) >= nn[i]
+                                        for i in range(D)) for q in range(Q))))
+            rho_l = fq.sum() + (1.0 if ddf else 0.0)
+            u_l = (fq @ cc) / rho_l
+            rho_s = rho_l if free in ("rho", "both") else rho_bc
+            u_s = u_l if free in ("u", "both") else u_bc
+            expect = _feq_np(cc, w, rho_s, u_s) - shift
+            got = np.array([out[O.addr(desc, xe, y, z, q, Q)] for q in range(Q)])
+            np.testing.assert_allclose(got, expect, rtol=0, atol=2e-15)
+    assert len(n_oob_seen) >= 2  # face nodes and edge/corner nodes both covered
+
+
+def test_reduce_moments_uniform_totals():
+    """A uniform equilibrium gives N * rho0 and N * rho0 * u0 (periodic box, rho0 != 1 so that
+    rho * u cannot pass for u); at rest with walls only the N_valid non-SOLID, non-EXCLUDE nodes
+    count, whatever the walls' bc holds."""
+    desc = U.make_desc("D3Q19", (16, 8, 8), storage=2)
+    f, bc, flags = O.alloc(desc)
+    rho0, u0 = 1.07, [0.03, -0.05, 0.02]
+    O.init_equilibrium(desc, f, rho0, u0)
+    tot = O.reduce_moments(desc, f, bc, flags)
+    N = 16 * 8 * 8
+    np.testing.assert_allclose(tot, [N * rho0] + [N * rho0 * v for v in u0], rtol=1e-13)
+    O.init_equilibrium(desc, f, rho0, [0.0, 0.0, 0.0])
+    O.set_exterior_wall(desc, flags, bc, "+Y", 1, [0.1, 0.0, 0.0], 3.0)
+    O.set_exterior_wall(desc, flags, bc, "-Y", 1, [0.0, 0.0, 0.0], 3.0)
+    ids = [O.addr(desc, x, 3, 4, 0, 1) for x in range(5, 9)]
+    flags[ids] |= 0x80  # EXCLUDE
+    tot = O.reduce_moments(desc, f, bc, flags)
+    n_valid = N - 2 * 16 * 8 - 4
+    assert abs(tot[0] - n_valid * rho0) < 1e-12 * n_valid
+    # the lid's moving-wall term carries momentum into the nodes under it: 6 w rho_w (c . u_w) per
+    # pulled link, summed over the 5 links that reach the lid from each of the 16 * 8 nodes
+    c, w, _ = O.lattice(3, 19)
+    inj = sum(6 * w[q] * 3.0 * c[q][0] * 0.1 * c[q][0] for q in range(19) if c[q][1] == -1)
+    mass_inj = sum(6 * w[q] * 3.0 * c[q][0] * 0.1 for q in range(19) if c[q][1] == -1)
+    assert abs(mass_inj) < 1e-16  # the +-x links cancel in the mass
+    assert abs(tot[1] - 16 * 8 * inj) < 1e-12
+    assert abs(tot[2]) < 1e-12 and abs(tot[3]) < 1e-12
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_reduce_moments_equal_field_sums(method):
+    """The FP64 totals equal numpy sums of the (separately pinned) macroscopic fields over the
+    nodes that are neither SOLID nor EXCLUDE, on a perturbed walled state."""
+    desc = U.make_desc("D3Q19", (16, 8, 8), storage=2, method=method)
+    f, bc, flags = U.perturbed_state(desc, seed=21, amp=2e-3)
+    rng = np.random.default_rng(3)
+    ids = rng.choice(f.size // 19, 40, replace=False)
+    flags[ids] |= 0x80
+    if method == 1:
+        f = U.db_to_ep_odd(desc, f)
+    rho, u = O.macroscopic(desc, f, bc, flags, after_odd=True)
+    tot = O.reduce_moments(desc, f, bc, flags, after_odd=True)
+    fl = np.zeros(rho.size, np.uint8)  # flags in row-major (x, y, z) node order
+    for i in range(rho.size):
+        x, y, z = i // 64, (i // 8) % 8, i % 8
+        fl[i] = flags[O.addr(desc, x, y, z, 0, 1)]
+    valid = ((fl & 0x0F) != 1) & ((fl & 0x80) == 0)
+    assert 0 < valid.sum() < rho.size
+    uu = u.reshape(-1, 3)
+    assert abs(tot[0] - math.fsum(rho[valid])) < 1e-12
+    for d in range(3):
+        assert abs(tot[1 + d] - math.fsum(rho[valid] * uu[valid, d])) < 1e-13
+
+
 def test_couette_profile():
     """D2Q9 channel, periodic in x, bottom wall at rest, top wall moving: the steady profile is linear
     between the halfway walls (y = 0.5 and y = ny - 1.5)."""

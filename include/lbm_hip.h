@@ -197,6 +197,12 @@ int lbm_set_exterior_wall(lbm_handle* h, uint8_t* flags, void* bc, const char* s
  * lbm_macroscopic and lbm_reduce_moments_f64 apply the same links when re-streaming.
  */
 int lbm_set_links(lbm_handle* h, const uint32_t* nodes, const float* qdist, uint32_t n_links);
+/* lbm_set_links with every link fraction qd >= 0 raised to q_clamp (0 <= q_clamp <= 1): the
+ * q_clamp argument of linkwise_bounceback_kernel / nodewise_bounceback_kernel
+ * (src/lbm/geometry/interpolated_BB.mojo:60,117, nodewise_interpolated_BB.mojo:69,151).
+ * lbm_set_links is this call with q_clamp = 0. */
+int lbm_set_links_clamped(lbm_handle* h, const uint32_t* nodes, const float* qdist, uint32_t n_links,
+                          float q_clamp);
 
 /*
  * Momentum-exchange force on the solid from the interpolated links of the last
@@ -292,9 +298,12 @@ int lbm_init_f_from_fields(lbm_handle* h, void* f, double rho, const void* u, co
  *                      0 = hardware round-robin order.
  *   LBM_OPT_STATIC_FLAGS: 1 = the caller promises that flags and bc do not change
  *                      while the same flags / bc buffers are passed: the per-tile
- *                      flag summary is rebuilt only when either pointer changes
- *                      (or this option is set again), not before every step.
- *                      0 (default) = rebuilt before every step.
+ *                      flag summary is rebuilt only when either pointer changes,
+ *                      this option is set again or lbm_set_exterior_wall runs on the
+ *                      handle, not before every step. A caller that edits flags or bc
+ *                      in place by other means (a copy into the same buffers, a new
+ *                      allocation that reuses a freed address) sets the option again
+ *                      before the next step. 0 (default) = rebuilt before every step.
  *   LBM_OPT_ADDR64:    1 = use the 64-bit addressing kernels (which buffers of 2 GiB and
  *                      more always use) for any size, so they can be checked on small grids;
  *                      0 (default) = only for buffers of 2 GiB and more.

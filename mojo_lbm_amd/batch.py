@@ -49,8 +49,9 @@ class CaseBatch:
         self.n_cases = len(self.taus)
 
     # -- setup
-    def set_case(self, c: int, flags=None, bc=None, links=None) -> None:
-        """flags [nx, ny, nz] uint8, bc [nx, ny, nz, D+1] float64, links = (row-major node ids, qdist)."""
+    def set_case(self, c: int, flags=None, bc=None, links=None, q_clamp: float = 0.0) -> None:
+        """flags [nx, ny, nz] uint8, bc [nx, ny, nz, D+1] float64, links = (row-major node ids, qdist),
+        each link fraction raised to q_clamp as lbm_set_links_clamped does."""
         if not 0 <= c < self.n_cases:
             raise IndexError(f"case {c} outside 0..{self.n_cases - 1}")
         n = int(np.prod(self.sizes[c]))
@@ -64,7 +65,10 @@ class CaseBatch:
         nl = 0
         if links is not None:
             nodes = np.ascontiguousarray(links[0], np.uint32)
-            qd = np.ascontiguousarray(links[1], np.float32)
+            qd = np.array(links[1], np.float32)
+            if not 0.0 <= q_clamp <= 1.0:
+                raise ValueError("q_clamp must lie in [0, 1]")
+            qd[qd >= 0] = np.maximum(qd[qd >= 0], np.float32(q_clamp))
             nl = len(nodes)
         _lib.check(self.L.lbm_batch_set_case(self.ptr, int(c), None if fl is None else fl.ctypes.data,
                                              None if b is None else b.ctypes.data,

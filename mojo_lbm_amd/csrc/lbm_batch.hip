@@ -159,6 +159,9 @@ static void put_r(std::vector<uint8_t>& v, size_t i, size_t rb, double x) {
 
 // Tiled host copies of flags, bc and links, copied to the device; summaries and case records.
 static int stage(Batch& b) {
+    // A captured graph holds the kernel variant (with or without links) and the link buffer
+    // pointers of the staging it was recorded after; either may change here.
+    free_graph(b);
     const int D = b.proto.D, Q = b.proto.Q;
     b.h_flags.assign(b.fl_bytes, (uint8_t)LBM_FLAG_SOLID);  // padding nodes are SOLID
     b.h_bc.assign(b.bc_elems * b.r_bytes, 0);

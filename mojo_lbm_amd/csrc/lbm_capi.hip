@@ -1,4 +1,5 @@
 // lbm_capi.hip -- the extern "C" boundary of liblbm_hip.so (declared in include/lbm_hip.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -322,6 +323,9 @@ int lbm_set_exterior_wall(lbm_handle* hh, uint8_t* flags, void* bc, const char* 
     }
     if (flag == LBM_FLAG_EQUILIBRIUM && !u_empty && std::isnan(rho))
         return set_error(LBM_ERR_INVALID, "For Equilibrium BC, if u is specified then the density must also be specified");
+    // the flags / bc change in place: a static-flags summary built from these buffers is stale
+    h.summary_flags = nullptr;
+    h.summary_bc = nullptr;
     const int n[3] = {h.d.nx, h.d.ny, h.d.nz};
     const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
     const int count = n[a1] * n[a2];
@@ -338,7 +342,13 @@ int lbm_set_exterior_wall(lbm_handle* hh, uint8_t* flags, void* bc, const char* 
 }
 
 int lbm_set_links(lbm_handle* hh, const uint32_t* nodes, const float* qdist, uint32_t n_links) {
+    return lbm_set_links_clamped(hh, nodes, qdist, n_links, 0.0f);
+}
+
+int lbm_set_links_clamped(lbm_handle* hh, const uint32_t* nodes, const float* qdist, uint32_t n_links,
+                          float q_clamp) {
     if (int rc = check_handle(hh)) return rc;
+    if (!(q_clamp >= 0.0f && q_clamp <= 1.0f)) return set_error(LBM_ERR_INVALID, "q_clamp must lie in [0, 1]");
     Handle& h = *H(hh);
     if (h.d.method != LBM_DOUBLE_BUFFER)
         return set_error(LBM_ERR_UNSUPPORTED, "interpolated bounce-back links need the double-buffer method");
@@ -356,7 +366,11 @@ int lbm_set_links(lbm_handle* hh, const uint32_t* nodes, const float* qdist, uin
     h.n_links = 0;
     h.force_valid = false;
     if (!n_links) return LBM_OK;
-    const size_t qbytes = (size_t)n_links * (h.d.Q - 1) * sizeof(float);
+    // q_dist = max(q_dist, q_clamp) on every link (qd >= 0); negative entries stay plain bounce-back
+    std::vector<float> qd(qdist, qdist + (size_t)n_links * (h.d.Q - 1));
+    for (float& v : qd)
+        if (v >= 0.0f) v = std::max(v, q_clamp);
+    const size_t qbytes = qd.size() * sizeof(float);
     if (hipMalloc((void**)&h.link_nodes, (size_t)n_links * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&h.link_q, qbytes) != hipSuccess)
         return set_error(LBM_ERR_HIP, "hipMalloc of the link records failed");
@@ -366,7 +380,7 @@ int lbm_set_links(lbm_handle* hh, const uint32_t* nodes, const float* qdist, uin
             return set_error(LBM_ERR_HIP, "hipMalloc of the force partials failed");
     }
     if (hipMemcpy(h.link_nodes, nodes, (size_t)n_links * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h.link_q, qdist, qbytes, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(h.link_q, qd.data(), qbytes, hipMemcpyHostToDevice) != hipSuccess)
         return set_error(LBM_ERR_HIP, "copy of the link records failed");
     h.n_links = n_links;
     return LBM_OK;

@@ -13,7 +13,7 @@
 // Populations are addressed through buffer resources: each lane carries a
 // 32-bit byte offset and the q-plane offset q * tile_cells rides in the scalar
 // soffset operand, so a load or store costs at most one vector add. Buffers of
-// 4 GB or more use a 64-bit pointer variant (BIG).
+// 2 GiB or more use a 64-bit pointer variant (BIG).
 #pragma once
 
 #include "lbm_physics.hpp"
@@ -443,17 +443,20 @@ __device__ __forceinline__ void halfway_bounce_back(const TileCtx<L>& t, const u
 
 // Interpolated bounce-back (linear Bouzidi, Firdaouss & Lallemand 2001; the formulas of
 // linkwise_bounceback_kernel, src/lbm/geometry/interpolated_BB.mojo:119-140, and
-// nodewise_bounceback_kernel, src/lbm/geometry/nodewise_interpolated_BB.mojo:154-178) on a
+// nodewise_bounceback_kernel, src/lbm/geometry/nodewise_interpolated_BB.mojo:140-178) on a
 // FLUID_SOLID_BOUNDARY node that has a link record: for each direction whose pull neighbour is
-// SOLID and whose link distance qd >= 0, the population entering along c_q is rebuilt from
-// a = f*(x, opp q) (already in fv[q] by halfway bounce-back) and
-//   qd <= 1/2: b = f*(x + c_q, opp q), f_q = (2 qd) a + (1 - 2 qd) b  (a when x + c_q is SOLID),
-//   qd >  1/2: b = f*(x, q),           f_q = (0.5 / qd) a + ((2 qd - 1) / (2 qd)) b;
-// the link's momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF) is added to force,
-// and the mass the interpolation adds, dm = sum (f_q - a), is taken from the rest
-// population so the node holds the mass halfway bounce-back gives it (the mass fix of
-// nodewise_interpolated_BB.mojo:170-176, which adds dm instead; subtracting it is what
-// conserves mass). Returns the rebuilt directions (they take no moving-wall term).
+// SOLID and whose link distance qd >= 0 (already raised to the handle's q_clamp), the population
+// entering along c_q is rebuilt from a = fv[q] = f*(x, opp q) (delivered by halfway bounce-back) and
+//   qd <= 1/2: b = fv[opp q],  f_q = (2 qd) a + (1 - 2 qd) b,
+//   qd >  1/2: b = f*(x, q),   f_q = (0.5 / qd) a + ((2 qd - 1) / (2 qd)) b.
+// fv[opp q] was pulled from x + c_q: f*(x + c_q, opp q) in the fluid, its halfway bounce-back in a
+// one-node gap, or the rebuilt value of the opposite link when that link came first. Links run in
+// the reference's order (its link bit q_ref rebuilds f_vec[opp q_ref]): opp(1), 1, opp(3), 3, ...
+// The link's momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF) is added to force, and the
+// mass the interpolation adds, dm = sum (f_q - a), is taken from the rest population so the node
+// holds the mass halfway bounce-back gives it (the mass fix of nodewise_interpolated_BB.mojo:170-176,
+// which adds dm instead; subtracting it is what conserves mass). Returns the rebuilt directions
+// (they take no moving-wall term).
 template <class L, class S, bool DDF, bool BIG>
 __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uint8_t* sfl, const Field<S, BIG>& f,
                                                   const uint32_t* __restrict__ link_nodes,
@@ -471,18 +474,15 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uin
     uint32_t done = 0;
     R dm = 0;  // mass the interpolation adds, taken back from the rest population
 #pragma unroll
-    for (int q = 1; q < Q; ++q) {
+    for (int k = 1; k < Q; ++k) {
+        const int q = opp(k);  // opp(1), 1, opp(3), 3, ...
         const R qd = (R)link_q[(size_t)lo * (Q - 1) + (q - 1)];
         if (!(qd >= (R)0) || t.pull_flag(sfl, q) != LBM_FLAG_SOLID) continue;
         const R a = fv[q];
         R fb;
         if (qd <= (R)0.5) {
-            if ((t.pull_flag(sfl, opp(q)) & 0x0F) == LBM_FLAG_SOLID) {
-                fb = a;
-            } else {
-                const R b = f.ld(t.nb(q, +1), (uint32_t)opp(q) * t.tc);
-                fb = (R)2 * qd * a + ((R)1 - (R)2 * qd) * b;
-            }
+            const R b = fv[opp(q)];
+            fb = (R)2 * qd * a + ((R)1 - (R)2 * qd) * b;
         } else {
             const R b = f.ld(t.fbase, (uint32_t)q * t.tc);
             fb = (R)0.5 / qd * a + ((R)2 * qd - (R)1) / ((R)2 * qd) * b;

@@ -70,7 +70,10 @@ def save(path, desc, f, bc, flags, step: int = 0, is_even: bool | None = None, m
         header["sections"][k] = {"offset": off, "bytes": int(v.numel()), "crc32": 0}
         off = _align(off + int(v.numel()))
     pinned = torch.empty(CHUNK, dtype=torch.uint8, pin_memory=torch.cuda.is_available())
-    with open(path, "wb") as fh:
+    # written beside the target and renamed over it once complete, so a crash mid-save leaves any
+    # earlier snapshot at `path` intact
+    tmp = f"{os.fspath(path)}.tmp"
+    with open(tmp, "wb") as fh:
         fh.write(b" " * HEADER_BYTES)
         for k, v in sections.items():
             sec = header["sections"][k]
@@ -88,6 +91,9 @@ def save(path, desc, f, bc, flags, step: int = 0, is_even: bool | None = None, m
             raise ValueError("snapshot metadata too large for the header")
         fh.seek(0)
         fh.write(text + b" " * (HEADER_BYTES - len(text)))
+        fh.flush()
+        os.fsync(fh.fileno())
+    os.replace(tmp, path)
     return header
 
 
@@ -107,37 +113,46 @@ def read_header(path) -> dict:
 
 def load(path, f=None, bc=None, flags=None, device="cuda", verify: bool = True):
     """Reads a snapshot into the given tensors (allocated when None) on `device`.
-    Returns (desc, f, bc, flags, header); raises on a CRC mismatch when verify is set."""
+    Returns (desc, f, bc, flags, header). With verify set, every section's CRC-32 is checked in
+    a first pass over the file, before anything is copied, so a corrupt snapshot raises and
+    leaves the destination tensors untouched."""
     import torch
 
     header = read_header(path)
     desc = desc_from_dict(header["desc"])
     outs = {"f": f, "bc": bc, "flags": flags}
-    pinned = torch.empty(CHUNK, dtype=torch.uint8, pin_memory=torch.cuda.is_available())
     size = os.path.getsize(path)
+    for k in ("f", "bc", "flags"):
+        sec = header["sections"][k]
+        if sec["offset"] + sec["bytes"] > size:
+            raise ValueError(f"snapshot truncated in section {k}")
+        if outs[k] is not None and _as_bytes_view(outs[k]).numel() != sec["bytes"]:
+            raise ValueError(f"{k} holds {_as_bytes_view(outs[k]).numel()} bytes, the snapshot has {sec['bytes']}")
+    if verify:
+        with open(path, "rb") as fh:
+            for k in ("f", "bc", "flags"):
+                sec = header["sections"][k]
+                fh.seek(sec["offset"])
+                crc = 0
+                for lo in range(0, sec["bytes"], CHUNK):
+                    crc = zlib.crc32(fh.read(min(CHUNK, sec["bytes"] - lo)), crc)
+                if crc != sec["crc32"]:
+                    raise ValueError(f"snapshot section {k} fails its CRC-32 check")
+    pinned = torch.empty(CHUNK, dtype=torch.uint8, pin_memory=torch.cuda.is_available())
     with open(path, "rb") as fh:
         for k in ("f", "bc", "flags"):
             sec = header["sections"][k]
-            if sec["offset"] + sec["bytes"] > size:
-                raise ValueError(f"snapshot truncated in section {k}")
             dst = outs[k]
             if dst is None:
                 dst = torch.empty(sec["bytes"], dtype=torch.uint8, device=device)
                 outs[k] = dst
             view = _as_bytes_view(dst)
-            if view.numel() != sec["bytes"]:
-                raise ValueError(f"{k} holds {view.numel()} bytes, the snapshot has {sec['bytes']}")
             fh.seek(sec["offset"])
-            crc = 0
             for lo in range(0, sec["bytes"], CHUNK):
                 n = min(CHUNK, sec["bytes"] - lo)
                 buf = fh.read(n)
-                if verify:
-                    crc = zlib.crc32(buf, crc)
                 pinned[:n].copy_(torch.frombuffer(bytearray(buf), dtype=torch.uint8))
                 view[lo:lo + n].copy_(pinned[:n])
-            if verify and crc != sec["crc32"]:
-                raise ValueError(f"snapshot section {k} fails its CRC-32 check")
     return desc, outs["f"], outs["bc"], outs["flags"], header
 
 

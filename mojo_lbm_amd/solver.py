@@ -83,5 +83,18 @@ class Solver:
             self.esoteric_pull_step(f, bc, flags, tau, False)
             assembly.f.mark_device_written()
 
+    def set_links(self, nodes, qdist, q_clamp: float = 0.0) -> None:
+        """Interpolated bounce-back links (geometry.sphere_links records) with each link fraction raised
+        to q_clamp, the argument of nodewise_bounceback_kernel (src/lbm/geometry/nodewise_interpolated_BB.mojo:69,151)."""
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        qdist = np.ascontiguousarray(qdist, np.float32)
+        _lib.check(self.L.lbm_set_links_clamped(self.handle.ptr, nodes.ctypes.data, qdist.ctypes.data, len(nodes),
+                                                float(q_clamp)))
+
+    def link_force(self, out) -> None:
+        """Momentum-exchange force of the last step into out (D device doubles): sum_force,
+        src/lbm/geometry/rigidstationary.mojo:155-169."""
+        _lib.check(self.L.lbm_link_force(self.handle.ptr, _ptr(out), _stream()))
+
     def synchronize(self) -> None:
         _lib.check(self.L.lbm_synchronize(_stream()))

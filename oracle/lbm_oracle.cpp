@@ -37,6 +37,7 @@
  *                           linear shear against a wall at half distance, one-sided edges
  *   LDC D2Q9 Re=100 ....... Ghia et al. centreline fixture (benchmarks/LDC/*), loose physics check
  */
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -434,15 +435,19 @@ template <class S, class T> struct Stepper {
 
     /* Interpolated bounce-back links (linear Bouzidi, Firdaouss & Lallemand 2001; the formulas of
      * linkwise_bounceback_kernel, src/lbm/geometry/interpolated_BB.mojo:119-140, and
-     * nodewise_bounceback_kernel, src/lbm/geometry/nodewise_interpolated_BB.mojo:154-178). Links live on
+     * nodewise_bounceback_kernel, src/lbm/geometry/nodewise_interpolated_BB.mojo:140-178). Links live on
      * FLUID_SOLID_BOUNDARY nodes: record r holds the node's flag-layout index link_nodes[r] (sorted) and,
      * for q = 1..Q-1, link_q[r*(Q-1) + q-1] = the fraction of the link from x towards the SOLID pull
-     * neighbour x - c_q that lies in the fluid (negative: plain halfway bounce-back). The population
-     * entering x along c_q is rebuilt from a = f*(x, opp q) (the one that hit the wall) and
-     *   qd <= 1/2: b = f*(x + c_q, opp q), f_q = (2 qd) a + (1 - 2 qd) b   (halfway bounce-back when
-     *              x + c_q is SOLID),
-     *   qd >  1/2: b = f*(x, q),           f_q = (0.5 / qd) a + ((2 qd - 1) / (2 qd)) b,
-     * and the link adds the momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF shifting) to the
+     * neighbour x - c_q that lies in the fluid (negative: plain halfway bounce-back), raised to q_clamp
+     * (nodewise_interpolated_BB.mojo:151). The population entering x along c_q is rebuilt from
+     * a = fv[q] = f*(x, opp q) (the one that hit the wall, delivered by halfway bounce-back) and
+     *   qd <= 1/2: b = fv[opp q],  f_q = (2 qd) a + (1 - 2 qd) b,
+     *   qd >  1/2: b = f*(x, q),   f_q = (0.5 / qd) a + ((2 qd - 1) / (2 qd)) b.
+     * fv[opp q] is the population pulled from x + c_q: f*(x + c_q, opp q) in the fluid, its halfway
+     * bounce-back f*(x, q) in a one-node gap, or -- when the opposite direction is itself a link --
+     * that link's rebuilt value if it came first. Links are taken in the reference's order: its link
+     * bit q_ref rebuilds f_vec[opp q_ref], so the rebuilt directions run opp(1), 1, opp(3), 3, ...
+     * Each link adds the momentum exchange c_{opp q} (a + f_q) (+ 2 w_q with DDF shifting) to the
      * force on the solid. The mass the interpolation adds, dm = sum (f_q - a), is taken from the rest
      * population (the reference's mass fix, nodewise_interpolated_BB.mojo:170-176, adds dm; subtracting
      * it is what leaves the node the mass halfway bounce-back gives it). Returns a bit mask of the
@@ -450,6 +455,7 @@ template <class S, class T> struct Stepper {
     const uint32_t* link_nodes = nullptr;
     const float* link_q = nullptr;
     uint32_t n_links = 0;
+    float q_clamp = 0.0f;
 
     uint32_t bouzidi(T* fv, const uint8_t pf[27], const S* f, const uint8_t* flags, const int idx[3],
                      double* force) const {
@@ -464,21 +470,17 @@ template <class S, class T> struct Stepper {
         if (lo >= n_links || link_nodes[lo] != node) return 0;
         uint32_t done = 0;
         T dm = 0;  // mass the interpolation adds, taken back from the rest population
-        for (int q = 1; q < L.Q; ++q) {
-            const T qd = (T)link_q[(size_t)lo * (L.Q - 1) + (q - 1)];
+        for (int k = 1; k < L.Q; ++k) {
+            const int q = L.opp[k];  // opp(1), 1, opp(3), 3, ...: the reference's link-bit order
+            T qd = (T)link_q[(size_t)lo * (L.Q - 1) + (q - 1)];
             if (!(qd >= (T)0) || pf[q] != LBM_FLAG_SOLID) continue;
+            qd = std::max(qd, (T)q_clamp);
             const int oq = L.opp[q];
-            const T a = Store<S, T>::load(f, g.addr(idx[0], idx[1], idx[2], oq, K()));
+            const T a = fv[q];
             T fb;
             if (qd <= (T)0.5) {
-                int y[3];
-                adjacent(g, L.D, idx, L.c[q], 1, y);
-                if (flag_is_solid(flags[g.addr(y[0], y[1], y[2], 0, 1)])) {
-                    fb = a;
-                } else {
-                    const T b = Store<S, T>::load(f, g.addr(y[0], y[1], y[2], oq, K()));
-                    fb = (T)2 * qd * a + ((T)1 - (T)2 * qd) * b;
-                }
+                const T b = fv[oq];
+                fb = (T)2 * qd * a + ((T)1 - (T)2 * qd) * b;
             } else {
                 const T b = Store<S, T>::load(f, g.addr(idx[0], idx[1], idx[2], q, K()));
                 fb = (T)0.5 / qd * a + ((T)2 * qd - (T)1) / ((T)2 * qd) * b;
@@ -659,6 +661,7 @@ template <class S, class T> struct Stepper {
 };
 
 int g_threads = 1;
+float g_q_clamp = 0.0f;  // lower bound of every link fraction (q_clamp of the reference's Bouzidi kernels)
 
 template <class S, class T>
 int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uint8_t* flags, double tau,
@@ -668,6 +671,7 @@ int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uin
     st.link_nodes = link_nodes;
     st.link_q = link_q;
     st.n_links = n_links;
+    st.q_clamp = g_q_clamp;
     const Geo& g = st.g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const T tau_t = (T)tau;
@@ -704,6 +708,7 @@ int macro_impl(const lbm_desc* d, T* rho_out, T* u_out, const S* f, const T* bc,
     st.link_nodes = link_nodes;
     st.link_q = link_q;
     st.n_links = n_links;
+    st.q_clamp = g_q_clamp;
     const Geo& g = st.g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const int D = st.L.D;
@@ -740,6 +745,7 @@ int reduce_impl(const lbm_desc* d, double* out, const S* f, const T* bc, const u
     st.link_nodes = link_nodes;
     st.link_q = link_q;
     st.n_links = n_links;
+    st.q_clamp = g_q_clamp;
     const Geo& g = st.g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const int D = st.L.D;
@@ -819,6 +825,7 @@ int q_impl(const lbm_desc* d, T* q_out, const T* u_in, int u_layout, const S* f,
     st.link_nodes = link_nodes;
     st.link_q = link_q;
     st.n_links = n_links;
+    st.q_clamp = g_q_clamp;
     const Geo& g = st.g;
     const long ncell = (long)g.nx * g.ny * g.nz;
     const int D = st.L.D;
@@ -966,6 +973,7 @@ void orc_mlp_closure(const float* params, const float* X, long n, double* nu) {
 }
 
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+void orc_set_q_clamp(float q) { g_q_clamp = q; }
 int orc_max_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -29,6 +29,7 @@ def lib():
         vp, i, d = C.c_void_p, C.c_int, C.c_double
         pd = C.POINTER(LbmDesc)
         L.orc_set_threads.argtypes = [i]
+        L.orc_set_q_clamp.argtypes = [C.c_float]
         L.orc_max_threads.restype = i
         L.orc_f16c_encode.argtypes = [C.c_float]
         L.orc_f16c_encode.restype = C.c_uint16
@@ -82,6 +83,11 @@ def _ok(rc: int, what: str):
 
 def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
+
+
+def set_q_clamp(q: float) -> None:
+    """Lower bound applied to every link fraction of the interpolated bounce-back (default 0)."""
+    lib().orc_set_q_clamp(float(q))
 
 
 def max_threads() -> int:

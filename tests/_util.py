@@ -128,10 +128,14 @@ class DeviceRun:
         t.cuda.synchronize()
         return out.cpu().numpy()
 
-    def set_links(self, nodes, qdist):
+    def set_links(self, nodes, qdist, q_clamp=None):
         nodes = np.ascontiguousarray(nodes, np.uint32)
         qdist = np.ascontiguousarray(qdist, np.float32)
-        _lib.check(self.L.lbm_set_links(self.h.ptr, nodes.ctypes.data, qdist.ctypes.data, len(nodes)))
+        if q_clamp is None:
+            _lib.check(self.L.lbm_set_links(self.h.ptr, nodes.ctypes.data, qdist.ctypes.data, len(nodes)))
+        else:
+            _lib.check(self.L.lbm_set_links_clamped(self.h.ptr, nodes.ctypes.data, qdist.ctypes.data, len(nodes),
+                                                    float(q_clamp)))
 
     def link_force(self):
         t = self.torch

@@ -70,6 +70,34 @@ def test_batch_matches_single_runs(gpu, grid_blocks, graph_steps, steps):
         assert np.allclose(forces[i], fref, rtol=1e-12, atol=1e-15)
 
 
+def test_batch_links_added_after_graph_run(gpu):
+    """A graph captured while no case had links must not be replayed once set_case adds them:
+    the replays would run the kernel without links and report a force nobody computed."""
+    from mojo_lbm_amd import get_D2Q9
+
+    print("case batch: links added after a graph run", flush=True)
+    lat = get_D2Q9()
+    cases = _small_cylinders(3, seed=11)
+    b = _batch(cases, lat, _config())
+    for i, c in enumerate(cases):
+        flags, bc, _ = c.arrays(lat)
+        b.set_case(i, flags, bc, None)  # halfway walls only
+    b.set_option(graph_steps=10)
+    b.init_equilibrium()
+    b.run(20)  # two graph replays without links
+    for i, c in enumerate(cases):
+        flags, bc, links = c.arrays(lat)
+        b.set_case(i, flags, bc, links)
+    f0 = [b.get_f(i).cpu().numpy() for i in range(len(cases))]
+    b.run(20)
+    forces = b.force()
+    for i, c in enumerate(cases):
+        ref, fref = _single(b, i, f0[i], c.tau, 20)
+        assert fref is not None
+        assert np.array_equal(b.get_f(i).cpu().numpy().view(np.uint8), ref.view(np.uint8)), f"case {i} differs"
+        assert np.allclose(forces[i], fref, rtol=1e-12, atol=1e-15)
+
+
 @pytest.mark.parametrize("collision", ["KBC", "TRT", "RLBM"])
 def test_batch_3d_les_from_perturbed_states(gpu, collision):
     import torch

@@ -74,3 +74,54 @@ def test_links_clear_and_errors(gpu):
     run.step_db(0.6)
     run2.step_db(0.6)
     assert np.array_equal(run.f_host(), run2.f_host())
+
+
+def _porous_case(lat, storage, ddf, seed):
+    """Periodic box with about a quarter of the nodes SOLID: every fluid node next to a SOLID node is
+    FLUID_SOLID_BOUNDARY with a link record whose fractions are random in [0, 1] (a few -1), so
+    one-node gaps, corners and opposite links on one node all occur."""
+    D, Q = U.LATTICES[lat]
+    n = (32, 16, 1) if D == 2 else (16, 16, 8)
+    desc = U.make_desc(lat, n, storage=storage, ddf=ddf)
+    f, bc, flags = U.perturbed_state(desc, seed=seed, amp=2e-3, walls=False)
+    rng = np.random.default_rng(seed)
+    c, _, _ = O.lattice(D, Q)
+    solid = rng.random(n) < 0.25
+    ids = U.index_map(desc, 1)[..., 0]
+    flags[ids[solid]] = 1
+    bc.reshape(-1, D + 1)[:, D] = 1.0
+    nodes, qd = [], []
+    for x, y, z in zip(*np.nonzero(~solid)):
+        row = np.full(Q - 1, -1.0, np.float32)
+        for q in range(1, Q):
+            p = ((x - c[q][0]) % n[0], (y - c[q][1]) % n[1], (z - c[q][2]) % n[2] if D == 3 else 0)
+            if solid[p] and rng.random() < 0.9:
+                row[q - 1] = rng.random()
+        if (row >= 0).any():
+            nodes.append(ids[x, y, z])
+            qd.append(row)
+    order = np.argsort(nodes)
+    nodes = np.array(nodes, np.uint32)[order]
+    qd = np.array(qd, np.float32)[order]
+    flags[nodes] = 3
+    return desc, f, bc, flags, nodes, qd
+
+
+@pytest.mark.parametrize("q_clamp", [None, 0.3])
+@pytest.mark.parametrize("lat,storage,ddf", [("D2Q9", 0, 0), ("D2Q9", 2, 1), ("D3Q19", 0, 1), ("D3Q27", 1, 1)])
+def test_links_porous_gaps_match_oracle(gpu, lat, storage, ddf, q_clamp):
+    print(f"case porous links {lat} storage={storage} ddf={ddf} q_clamp={q_clamp}", flush=True)
+    desc, f, bc, flags, nodes, qdist = _porous_case(lat, storage, ddf, seed=3 + storage)
+    run = U.DeviceRun(desc, f, bc, flags)
+    run.set_links(nodes, qdist, q_clamp=q_clamp)
+    O.set_q_clamp(q_clamp or 0.0)
+    try:
+        a, b = f.copy(), f.copy()
+        for step in range(3):
+            run.step_db(0.7)
+            force = O.step_db_links(desc, b, a, bc, flags, 0.7, nodes, qdist)
+            a, b = b, a
+            assert np.array_equal(run.f_host().view(np.uint8), a.view(np.uint8)), f"differs after step {step + 1}"
+    finally:
+        O.set_q_clamp(0.0)
+    assert np.allclose(run.link_force(), force, rtol=1e-12, atol=1e-12)

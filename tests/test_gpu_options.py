@@ -44,6 +44,40 @@ def test_static_flags_follow_a_new_flags_buffer(gpu):
     assert np.array_equal(run.f_host(), ref)
 
 
+def test_static_flags_follow_in_place_edits(gpu):
+    """Flags / bc edited in place in the same buffers: lbm_set_exterior_wall on the handle drops the
+    static summary by itself; after any other in-place edit, setting the option again does."""
+    import ctypes as C
+
+    import torch
+
+    desc = U.make_desc("D3Q19", (16, 16, 16))
+    f, bc, flags = U.perturbed_state(desc, seed=12, walls=False)
+    run = U.DeviceRun(desc, f, bc, flags)
+    _lib.check(run.L.lbm_set_option(run.h.ptr, _lib.OPT_STATIC_FLAGS, 1))
+    run.step_db(0.7)  # summary built: every tile interior
+    u = (C.c_double * 3)(0.05, 0.0, 0.0)
+    _lib.check(run.L.lbm_set_exterior_wall(run.h.ptr, run.flags.data_ptr(), run.bc.data_ptr(), b"+Y", 1, u,
+                                           1.0, run.stream))
+    run.step_db(0.7)
+    flags2, bc2 = flags.copy(), bc.copy()
+    O.set_exterior_wall(desc, flags2, bc2, "+Y", 1, [0.05, 0.0, 0.0], 1.0)
+    # the same two buffers as the device run, so SOLID nodes (never written) hold the same values
+    a, b = f.copy(), f.copy()
+    O.step_db(desc, b, a, bc, flags, 0.7)
+    O.step_db(desc, a, b, bc2, flags2, 0.7)
+    assert np.array_equal(run.f_host(), a)
+    # an obstacle copied into the same flags buffer, then the option set again
+    flags3 = flags2.copy()
+    amap = U.index_map(desc, 1)[..., 0]
+    flags3[amap[5:9, 4:8, 6:10].ravel()] = 1
+    run.flags.copy_(torch.from_numpy(flags3))
+    _lib.check(run.L.lbm_set_option(run.h.ptr, _lib.OPT_STATIC_FLAGS, 1))
+    run.step_db(0.7)
+    O.step_db(desc, b, a, bc2, flags3, 0.7)
+    assert np.array_equal(run.f_host(), b)
+
+
 ADDR64_CASES = [("D2Q9", 0, 0, 0), ("D2Q9", 1, 1, 1), ("D3Q19", 0, 0, 0), ("D3Q19", 1, 1, 1), ("D3Q19", 2, 0, 1),
                 ("D3Q27", 0, 1, 0), ("D3Q27", 2, 0, 0)]
 

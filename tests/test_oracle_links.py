@@ -125,3 +125,71 @@ def test_bouzidi_conserves_mass_in_a_closed_box(lat):
         a, b = b, a
     m1 = O.reduce_moments(desc, a, bc, flags, links=(nodes, qdist))[0]
     assert abs(m1 - m0) < 1e-12 * m0
+
+
+def _gap_case(qd1, qd2, ddf=0, seed=4):
+    """D2Q9 FP64, node (5, 6) between SOLID nodes (4, 6) and (6, 6): links for q = 1 (pull neighbour
+    x - c_1 = (4, 6)) and q = 2 (pull neighbour (6, 6)), every other neighbour fluid."""
+    desc = U.make_desc("D2Q9", (16, 16, 1), storage=2, ddf=ddf)
+    f, bc, flags = O.alloc(desc)
+    f[:] = np.random.default_rng(seed).uniform(0.02, 0.2, f.size)
+    bc[:] = 0.0
+    for x in (4, 6):
+        flags[O.addr(desc, x, 6, 0, 0, 1)] = 1
+        bc[O.addr(desc, x, 6, 0, 2, 3)] = 1.0
+    node = O.addr(desc, 5, 6, 0, 0, 1)
+    flags[node] = 3
+    qdist = np.full((1, 8), -1.0, np.float32)
+    qdist[0, 0], qdist[0, 1] = qd1, qd2
+    return desc, f, bc, flags, np.array([node], np.uint32), qdist
+
+
+@pytest.mark.parametrize("qd1,qd2", [(0.3, 0.4), (0.2, 0.7), (0.8, 0.25), (0.6, 0.9)])
+def test_bouzidi_one_node_gap_follows_reference(qd1, qd2):
+    """In a one-node gap both links of the pair act, in the order of the reference's link bits
+    (nodewise_interpolated_BB.mojo:140-178): bit q_ref = 1 rebuilds f_vec[2] first, with
+    f_at_xff = f_vec[1] (the halfway bounce-back of the other wall); bit 2 then rebuilds f_vec[1]
+    with f_at_xff = the just rebuilt f_vec[2]. qd > 1/2 reads f_out_of_wall = f(x, opp q_ref)
+    from memory. The rest population gives back the interpolated mass (dm subtracted)."""
+    desc, f, bc, flags, nodes, qdist = _gap_case(qd1, qd2)
+    out = f.copy()
+    O.step_db_links(desc, out, f, bc, flags, 1e30, nodes, qdist)
+    c, w, opp = O.lattice(2, 9)
+    x = (5, 6)
+    fin = lambda xx, yy, q: f[O.addr(desc, xx % 16, yy % 16, 0, q, 9)]
+    fv = np.array([fin(x[0] - c[q][0], x[1] - c[q][1], q) for q in range(9)])
+    fv[1], fv[2] = fin(*x, 2), fin(*x, 1)  # halfway bounce-back from both walls
+    dm = 0.0
+    for q_ref in (1, 2):  # reference bit order; bit q_ref rebuilds f_vec[opp q_ref] with link qd[opp q_ref]
+        o = opp[q_ref]
+        qd = float(np.float32(qd1 if o == 1 else qd2))
+        a = fv[o]
+        if qd > 0.5:
+            fb = 0.5 / qd * a + (2 * qd - 1) / (2 * qd) * fin(*x, o)
+        else:
+            fb = 2 * qd * a + (1 - 2 * qd) * fv[q_ref]
+        dm += fb - a
+        fv[o] = fb
+    fv[0] -= dm
+    got = np.array([out[O.addr(desc, *x, 0, q, 9)] for q in range(9)])
+    np.testing.assert_allclose(got, fv, rtol=0, atol=1e-16)
+
+
+def test_q_clamp_raises_small_fractions():
+    """q_clamp (nodewise_interpolated_BB.mojo:151, q_dist = max(q_dist, q_clamp)): links below the
+    clamp act as links at the clamp, links above it are untouched."""
+    desc, f, bc, flags, nodes, qdist = _gap_case(0.05, 0.6)
+    ref = f.copy()
+    q2 = qdist.copy()
+    q2[0, 0] = 0.2
+    O.step_db_links(desc, ref, f, bc, flags, 0.8, nodes, q2)
+    O.set_q_clamp(0.2)
+    try:
+        out = f.copy()
+        O.step_db_links(desc, out, f, bc, flags, 0.8, nodes, qdist)
+    finally:
+        O.set_q_clamp(0.0)
+    np.testing.assert_array_equal(out, ref)
+    plain = f.copy()
+    O.step_db_links(desc, plain, f, bc, flags, 0.8, nodes, qdist)
+    assert not np.array_equal(plain, ref)

@@ -27,6 +27,30 @@ def test_round_trip_on_the_host(tmp_path):
     assert hdr["sections"]["f"]["offset"] % 4096 == 0 and hdr["sections"]["flags"]["offset"] % 4096 == 0
 
 
+def test_save_replaces_the_file_only_when_complete(tmp_path, monkeypatch):
+    """A save that fails part way leaves the earlier snapshot at the path readable."""
+    desc = U.make_desc("D2Q9", (32, 32))
+    f, bc, flags = _state(desc, 2)
+    p = tmp_path / "s.lbm"
+    snapshot.save(p, desc, f, bc, flags, step=10)
+    real = snapshot.zlib.crc32
+    calls = {"n": 0}
+
+    def failing_crc(buf, crc=0):
+        calls["n"] += 1
+        if calls["n"] == 2:
+            raise OSError("disk full")
+        return real(buf, crc)
+
+    monkeypatch.setattr(snapshot.zlib, "crc32", failing_crc)
+    with pytest.raises(OSError):
+        snapshot.save(p, desc, f + 1, bc, flags, step=20)
+    monkeypatch.setattr(snapshot.zlib, "crc32", real)
+    f2, bc2, fl2 = np.zeros_like(f), np.zeros_like(bc), np.zeros_like(flags)
+    _, _, _, _, h = snapshot.load(p, f2, bc2, fl2, device="cpu")
+    assert h["step"] == 10 and np.array_equal(f2, f)
+
+
 def test_corruption_and_version_are_detected(tmp_path):
     desc = U.make_desc("D2Q9", (32, 32))
     f, bc, flags = _state(desc, 1)
@@ -35,8 +59,11 @@ def test_corruption_and_version_are_detected(tmp_path):
     raw = bytearray(p.read_bytes())
     raw[hdr["sections"]["bc"]["offset"] + 5] ^= 0x10
     p.write_bytes(bytes(raw))
+    dst = (np.full_like(f, 7), np.full_like(bc, 7), np.full_like(flags, 7))
     with pytest.raises(ValueError, match="CRC"):
-        snapshot.load(p, np.zeros_like(f), np.zeros_like(bc), np.zeros_like(flags), device="cpu")
+        snapshot.load(p, *dst, device="cpu")
+    # the corrupt section is the second one: nothing, not even the intact first section, was copied
+    assert np.all(dst[0] == 7) and np.all(dst[1] == 7) and np.all(dst[2] == 7)
     h = json.loads(raw[:4096].decode())
     h["version"] = 99
     text = json.dumps(h).encode()

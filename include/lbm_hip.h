@@ -251,6 +251,18 @@ int lbm_mlp_loss_grad(const float* params_dev, const float* X_dev, const float* 
                       float* grad_dev, double* loss_dev, void* workspace_dev, void* stream);
 int lbm_mlp_adam(float* params_dev, float* m_dev, float* v_dev, const float* grad_dev, int32_t t,
                  float lr, float beta1, float beta2, float eps, void* stream);
+/*
+ * lbm_mlp_loss_grad without a workspace argument and without any allocation: no hipMalloc, hipFree
+ * or allocating library call, for callers that cannot allocate (code under HIP graph capture,
+ * training loops in harnesses that forbid allocation). No counterpart in the reference. Its scratch
+ * is a static device array of 5,779,456 bytes per GPU (the gradient partials of the largest launch,
+ * 1024 workgroups, so every n > 0 fits), kept until the process ends; its address on each device is
+ * looked up on the first call made with that device current. Same kernels as lbm_mlp_loss_grad, same
+ * results bit for bit. Every call on one GPU shares that scratch: calls that overlap on one GPU (two
+ * streams, two host threads) race and must be serialised by the caller.
+ */
+int lbm_mlp_loss_grad_noalloc(const float* params_dev, const float* X_dev, const float* y_dev, int64_t n,
+                              float* grad_dev, double* loss_dev, void* stream);
 
 /*
  * Training data of the closure (device arrays; no reference counterpart).

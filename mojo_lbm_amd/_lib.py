@@ -67,6 +67,7 @@ SIGNATURES = {
     "lbm_mlp_closure": ([_vp, _vp, C.c_int64, _vp, _vp], _i),
     "lbm_mlp_workspace_bytes": ([C.c_int64], _sz),
     "lbm_mlp_loss_grad": ([_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp], _i),
+    "lbm_mlp_loss_grad_noalloc": ([_vp, _vp, _vp, C.c_int64, _vp, _vp, _vp], _i),
     "lbm_mlp_adam": ([_vp, _vp, _vp, _vp, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, _vp], _i),
     "lbm_box_filter": ([_vp, _vp, _i, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                         C.c_int32, C.c_int32, _vp, _vp], _i),

@@ -11,6 +11,7 @@
 // deterministic.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "lbm_launch.hpp"
@@ -19,6 +20,15 @@ namespace lbm {
 
 constexpr int MLP_CHUNK = 64;
 constexpr int MLP_OWN = (MLP_NPARAM + 63) / 64;  // entries per lane
+constexpr int MLP_MAX_BLOCKS = 1024;             // workgroups of mlp_grad_kernel (grid-stride beyond)
+
+// Scratch of lbm_mlp_loss_grad_noalloc: the per-workgroup gradient partials and loss partials of
+// the largest gradient launch (MLP_MAX_BLOCKS workgroups), 5,779,456 bytes per GPU, held until the
+// process ends. Host code never uses this name as an address: it reads each device's address once
+// with hipGetSymbolAddress (scratch_address below) and passes that.
+constexpr size_t MLP_SCRATCH_BYTES = (size_t)MLP_MAX_BLOCKS * MLP_NPARAM * sizeof(float) +
+                                     (size_t)MLP_MAX_BLOCKS * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[MLP_SCRATCH_BYTES];
 
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
 
@@ -214,7 +224,7 @@ int lbm_mlp_forward(const float* params_dev, const float* X_dev, int64_t n, floa
 
 size_t lbm_mlp_workspace_bytes(int64_t n) {
     if (n <= 0) return 0;
-    const int64_t blocks = std::min<int64_t>((n + MLP_CHUNK - 1) / MLP_CHUNK, 1024);
+    const int64_t blocks = std::min<int64_t>((n + MLP_CHUNK - 1) / MLP_CHUNK, MLP_MAX_BLOCKS);
     return (size_t)blocks * MLP_NPARAM * sizeof(float) + (size_t)blocks * sizeof(double);
 }
 
@@ -222,7 +232,7 @@ int lbm_mlp_loss_grad(const float* params_dev, const float* X_dev, const float* 
                       double* loss_dev, void* workspace_dev, void* stream) {
     if (!params_dev || !X_dev || !y_dev || !grad_dev || !workspace_dev || n <= 0)
         return set_error(LBM_ERR_INVALID, "bad closure gradient arguments");
-    const int blocks = (int)std::min<int64_t>((n + MLP_CHUNK - 1) / MLP_CHUNK, 1024);
+    const int blocks = (int)std::min<int64_t>((n + MLP_CHUNK - 1) / MLP_CHUNK, MLP_MAX_BLOCKS);
     float* partial = (float*)workspace_dev;
     double* loss_part = (double*)(partial + (size_t)blocks * MLP_NPARAM);
     hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks), dim3(64), 0, (hipStream_t)stream, params_dev, X_dev, y_dev, n,
@@ -231,6 +241,55 @@ int lbm_mlp_loss_grad(const float* params_dev, const float* X_dev, const float* 
     hipLaunchKernelGGL(mlp_grad_finish, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, (hipStream_t)stream, partial,
                        loss_part, blocks, n, grad_dev, loss_dev);
     return check_launch("mlp_grad_finish");
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per device (with that device
+// current) on the first call there.
+int scratch_address(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0)
+        return set_error(LBM_ERR_NO_DEVICE, "lbm_mlp_loss_grad_noalloc: no current HIP device");
+    if (dev >= SCRATCH_MAX_DEVICES)
+        return set_error(LBM_ERR_UNSUPPORTED, "lbm_mlp_loss_grad_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(LBM_ERR_HIP, std::string("hipGetSymbolAddress of the gradient scratch failed: ") +
+                                              hipGetErrorString(e));
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_mlp_loss_grad_noalloc(const float* params_dev, const float* X_dev, const float* y_dev, int64_t n,
+                              float* grad_dev, double* loss_dev, void* stream) {
+    if (!params_dev || !X_dev || !y_dev || !grad_dev || n <= 0)
+        return set_error(LBM_ERR_INVALID, "bad closure gradient arguments");
+    const size_t need = lbm_mlp_workspace_bytes(n);
+    if (need > MLP_SCRATCH_BYTES)
+        return set_error(LBM_ERR_INVALID, "lbm_mlp_loss_grad_noalloc: " + std::to_string(need) +
+                                              " bytes of scratch needed, the static scratch holds " +
+                                              std::to_string(MLP_SCRATCH_BYTES));
+    void* ws = nullptr;
+    if (int rc = scratch_address(&ws)) return rc;
+    // Every workgroup of the gradient kernel overwrites all of its partials before the finishing
+    // kernel reads them, so nothing in the scratch has to start at zero.
+    return lbm_mlp_loss_grad(params_dev, X_dev, y_dev, n, grad_dev, loss_dev, ws, stream);
 }
 
 int lbm_mlp_closure(lbm_handle* hh, const float* X_dev, int64_t n, float* nu_dev, void* stream) {

@@ -230,7 +230,7 @@ def accuracy_study(N, checkpoints, u_lid=0.1, Re=1000.0):
         del ref
     line = {"study": f"D3Q19 lid-driven cube {N}^3, u_lid={u_lid}, Re={Re:g}, tau={tau:.4f}: error vs the FP64 run "
                      "of the same case and method (fluid nodes)", "results": results}
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def cylinder_batch(steps, autotune_steps=100, seed=2024, compare=False, tile=None):
@@ -429,6 +429,10 @@ def main():
                     help="comma-separated checkpoints: run the Float16C / FP32 vs FP64 accuracy study instead")
     ap.add_argument("--cylinders", type=int, default=20000,
                     help="steps of the 64-case batched cylinder run reported under 'cylinder_batch' (0: skip)")
+    ap.add_argument("--accuracy-steps", default="1000,5000,20000",
+                    help="checkpoints of the Float16C / FP32 vs FP64 study folded into the default line ('' skips it)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip running every cylinder case alone after the batch")
     ap.add_argument("--cylinder-compare", action="store_true",
                     help="run only the batched cylinder cases, then every case alone, and compare")
     ap.add_argument("--cylinder-tile", default=None,
@@ -451,7 +455,7 @@ def main():
               flush=True)
         return
     if args.accuracy:
-        accuracy_study(args.n, [int(s) for s in args.accuracy.split(",")])
+        print(json.dumps(accuracy_study(args.n, [int(s) for s in args.accuracy.split(",")])), flush=True)
         return
     if args.calibrate:
         import torch
@@ -484,19 +488,20 @@ def main():
                        storage=storage, ddf_shift=ddf, les=0, include_moving_boundary=1, included_bcs=3,
                        smagorinsky_cs=0.1)
 
+    tau_f16c = 0.5 + 3.0 * u_lid * N / 1000.0  # the Float16C case is quoted at Re = 1000
     variants = {
-        "fp32": (0, 0, 0, 4, True),      # storage, method, ddf, bytes per population, walls
-        "f16c": (1, 0, 1, 2, True),
-        "f16c_ep": (1, 1, 1, 2, True),
-        "fp32_ep": (0, 1, 0, 4, True),
+        "fp32": (0, 0, 0, 4, True, tau),      # storage, method, ddf, bytes per population, walls, tau
+        "f16c": (1, 0, 1, 2, True, tau_f16c),
+        "f16c_ep": (1, 1, 1, 2, True, tau_f16c),
+        "fp32_ep": (0, 1, 0, 4, True, tau),
         # the reference's own benchmark box: no walls, fully periodic (kernels/benchmark/double_buffer.mojo:88-91)
-        "fp32_periodic": (0, 0, 0, 4, False),
-        "f16c_periodic": (1, 0, 1, 2, False),
+        "fp32_periodic": (0, 0, 0, 4, False, tau),
+        "f16c_periodic": (1, 0, 1, 2, False, tau_f16c),
     }
     names = [args.only] if args.only else list(variants)
     results = {}
     for name in names:
-        storage, method, ddf, s, walls = variants[name]
+        storage, method, ddf, s, walls, tau_v = variants[name]
         d = desc_for(storage, method, ddf)
         run = make_run(d, u_lid, walls)
         # flags and bc are static in this benchmark: the flag summary is built once
@@ -506,11 +511,12 @@ def main():
         if args.step_lanes is not None:
             _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_STEP_LANES, args.step_lanes))
         steps = args.steps if (name == "fp32" or args.only) else max(100, args.steps // 2)
-        t = time_run(run, tau, steps, args.warmup)
+        t = time_run(run, tau_v, steps, args.warmup)
         B = algorithmic_bytes(19, N, N, N, s, walls)
         ms = t / steps * 1e3
         results[name] = {
             "ms_per_step": ms,
+            "tau": tau_v,
             "mlups": N ** 3 * steps / t / 1e6,
             "gbs": B / (t / steps) / 1e9,
             "bytes_per_step": B,
@@ -555,12 +561,19 @@ def main():
             "traffic": traffic,
         },
         "variants": {k: {"mlups": round(v["mlups"], 1), "ms_per_step": round(v["ms_per_step"], 5),
-                         "roofline_frac": round(v["gbs"] / HBM_PEAK_GBS, 4)} for k, v in results.items()},
+                         "roofline_frac": round(v["gbs"] / HBM_PEAK_GBS, 4), "tau": round(v["tau"], 4)}
+                     for k, v in results.items()},
     }
     if "fp32" in results and "f16c" in results:
         line["variants"]["f16c_over_fp32"] = round(results["f16c"]["mlups"] / results["fp32"]["mlups"], 3)
     if args.cylinders and not args.only:
-        cyl = cylinder_batch(args.cylinders)
+        # BL-cfg4: the 64 cases batched, then every case alone (populations and Cd / Cl compared)
+        cyl = cylinder_batch(args.cylinders, compare=not args.no_compare)
+        if "compare" in cyl:
+            cmp_ = cyl["compare"]
+            cmp_.pop("per_case", None)
+            print(f"# cylinder batch vs single: {cmp_['bitwise_equal_cases']}/64 bitwise equal, "
+                  f"max |dCd| {cmp_['max_abs_dcd']:.3g}, max |dCl| {cmp_['max_abs_dcl']:.3g}", file=sys.stderr, flush=True)
         print(f"# cylinder batch: {cyl['mlups']:.0f} MLUPs over {cyl['cases']} cases, tile {cyl['tile']}, "
               f"grid {cyl['grid_blocks']}, {cyl['roofline_frac']:.1%} of 8 TB/s", file=sys.stderr, flush=True)
         line["cylinder_batch"] = cyl
@@ -571,6 +584,9 @@ def main():
         les_t = les_step_times()
         print(f"# les 128^3: {les_t['ms_per_step']}", file=sys.stderr, flush=True)
         line["les_closure"] = les_t
+        if args.accuracy_steps:
+            # BL-cfg3: Float16C + DDF (DB and esoteric) and FP32 vs FP64 runs of the Re = 1000 cube
+            line["accuracy"] = accuracy_study(N, [int(v) for v in args.accuracy_steps.split(",")])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(desc_for(0, 0, 0), tau)
     else:

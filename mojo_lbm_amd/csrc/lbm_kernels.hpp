@@ -49,6 +49,7 @@ template <class R> struct MacroParams {
     R* rho;
     R* u;
     double* partial;           // reduction mode: per-tile (D+1) sums
+    const uint32_t* summary;   // per-tile region masks (interior tiles skip the flags), or NULL
     const uint32_t* link_nodes;  // interpolated bounce-back records (double buffer), or null
     const float* link_q;
     uint32_t n_links;
@@ -60,6 +61,7 @@ template <class R> struct MacroParams {
     R* qout;                   // Q-criterion output (row-major x, y, z)
     R tau;                     // Q-criterion strain-rate tau
     uint32_t su_offset;        // Q-criterion: byte offset of the LDS velocity tile
+    uint32_t stage_offset;     // byte offset of the LDS output staging buffer (fields, Q)
     Geo g;
 };
 
@@ -719,7 +721,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
 // right_dx), right_weight = 1 - left_weight. Neighbours outside the grid are skipped (no
 // periodic wrap), which leaves a single valid side with weight 0. Velocities come from the
 // LDS velocity tile su (D values per halo cell, the flag tile's cell order).
-template <class R, int D>
+template <class R, int D, bool FLAGS = true>
 __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3], const R* su, const uint8_t* sfl,
                                          int center, const int (&lstride)[3], int comp, int axis) {
     const R uc = su[center * D + comp];
@@ -728,7 +730,7 @@ __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3],
     if (left_ok) {
         const int c = center - lstride[axis];
         const R ua = su[c * D + comp];
-        const R gdx = sfl[c] == LBM_FLAG_SOLID ? (R)0.5 : (R)1;
+        const R gdx = (FLAGS && sfl[c] == LBM_FLAG_SOLID) ? (R)0.5 : (R)1;
         left_grad = (uc - ua) * ((R)1 / gdx);
         left_dx = gdx;
     }
@@ -736,7 +738,7 @@ __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3],
     if (right_ok) {
         const int c = center + lstride[axis];
         const R ua = su[c * D + comp];
-        const R gdx = sfl[c] == LBM_FLAG_SOLID ? (R)0.5 : (R)1;
+        const R gdx = (FLAGS && sfl[c] == LBM_FLAG_SOLID) ? (R)0.5 : (R)1;
         right_grad = (ua - uc) * ((R)1 / gdx);
         right_dx = gdx;
     }
@@ -780,23 +782,16 @@ __device__ __forceinline__ void load_velocity_tile(const MacroParams<R>& p, int 
     }
 }
 
-// Row-major outputs of one tile. Lanes hold nodes in the tile's column-major order (x
-// fastest), which a row-major (x, y, z) field would receive as 64 scattered words per
-// wave; instead the tile's values go to LDS in row-major order (K values per node, either
-// interleaved per node or one plane per value) and each lane then writes the next
-// element, so a wave writes runs of tz * K (interleaved) or tz (planar) consecutive
-// words. The LDS buffer holds tile_cells * K values.
+// Row-major outputs of one tile. Lanes hold nodes in the tile's column-major order (x fastest),
+// which a row-major (x, y, z) field would receive as scattered words; the values go to LDS in
+// row-major order of the tile (K values per node, interleaved or one plane per value) and each
+// lane then writes the next element, so a wave writes runs of tz * K (interleaved) or tz (planar)
+// consecutive words.
 template <class R, int K>
-__device__ __forceinline__ void store_rowmajor(const Geo& g, int bx, int by, int bz, int tid, R* lds, const R (&v)[K],
-                                               R* out, bool planar) {
-    const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
-    const int tc = tx * ty * tz;
-    const int lx = tid & (tx - 1), ly = (tid >> g.txb) & (ty - 1), lz = tid >> (g.txb + g.tyb);
-    const int kr = (lx * ty + ly) * tz + lz;
-    __syncthreads();  // the buffer's previous contents have been written out
-#pragma unroll
-    for (int k = 0; k < K; ++k) lds[planar ? k * tc + kr : kr * K + k] = v[k];
-    __syncthreads();
+__device__ __forceinline__ void write_rowmajor(const Geo& g, int bx, int by, int bz, int tid, const R* lds, R* out,
+                                               bool planar) {
+    const int ty = 1 << g.tyb, tz = 1 << g.tzb;
+    const int tc = g.tile_cells;
     const size_t ncell = (size_t)g.nx * g.ny * g.nz;
 #pragma unroll
     for (int pass = 0; pass < K; ++pass) {
@@ -809,9 +804,17 @@ __device__ __forceinline__ void store_rowmajor(const Geo& g, int bx, int by, int
     }
 }
 
-// Tile of a macro_kernel workgroup: the step's order (x fastest, XCD-contiguous), so the
-// pulls share their halo lines in L2 as the step's do. (Walking tiles z-fastest, so that the
-// workgroups filling one row-major output line run together, measured 4-6 % slower at 256^3.)
+// Position of a node of the tile in the row-major staging order of write_rowmajor.
+__device__ __forceinline__ int rowmajor_slot(const Geo& g, int tid) {
+    const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
+    const int lx = tid & (tx - 1), ly = (tid >> g.txb) & (ty - 1), lz = tid >> (g.txb + g.tyb);
+    return (lx * ty + ly) * tz + lz;
+}
+
+// Tile of a macro_kernel workgroup: the step's order (x fastest, XCD-contiguous), so the pulls
+// share their halo lines in L2 as the step's do. (Walking tiles z-fastest, or in groups of four
+// tiles adjacent in z, so that the workgroups filling one row-major output line run together,
+// measured 4-16 % slower at 256^3.)
 __device__ __forceinline__ uint32_t macro_tile(const Geo& g) { return (uint32_t)tile_of_block(g); }
 
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125), the FP64 moment
@@ -819,20 +822,29 @@ __device__ __forceinline__ uint32_t macro_tile(const Geo& g) { return (uint32_t)
 // BC (the BCs take no tau), then moments, or Q = 0.25 |omega|^2 - 0.5 |S|^2 with omega from
 // finite differences of the velocity field and S from the re-streamed populations.
 // Solid nodes (raw flag == SOLID) report bc in the fields and 0 as Q.
+// Interior tiles (own nodes and halo all FLUID, from the tile summary) need no flags: no LDS flag
+// tile, no barrier, no boundary work.
 template <class L, class S, int MODE, bool DDF, int KIND, bool BIG>
-__global__ void __launch_bounds__(MaxThreads<L::Q>::value) __attribute__((amdgpu_waves_per_eu(6, 8)))
-    macro_kernel(MacroParams<typename S::R> p) {
+__global__ void __launch_bounds__(MaxThreads<L::Q>::value)
+    __attribute__((amdgpu_waves_per_eu(KIND == MK_FIELDS ? 8 : 6, 8))) macro_kernel(MacroParams<typename S::R> p) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
     extern __shared__ uint8_t sfl[];
     __shared__ double red[D + 1][16];
+    R* stage = (R*)(sfl + p.stage_offset);  // output staging: FIELDS rho plane + u, QCRIT Q
     TileCtx<L> t;
     t.init_pos(p.g, macro_tile(p.g), 0, p.g.tzb);
     const Field<S, BIG> f(p.f, p.f_bytes);
-    t.fetch_flags(p.g, p.flags);
+    const bool boundary = p.summary ? (boundary_tile<L::D>(p.g, p.summary, t.bx, t.by, t.bz) & 1u) != 0u : true;
+    if (boundary) t.fetch_flags(p.g, p.flags);
     R fv[Q];
     load_populations<L, S, MODE, BIG>(t, f, fv);
-    t.commit_flags(p.g, sfl);
+    if (boundary) {
+        t.commit_flags(p.g, sfl);
+    } else {
+        t.nonfluid = 0;
+        t.own = LBM_FLAG_FLUID;
+    }
     if (t.nonfluid) {
         uint32_t rebuilt = 0;
         if (MODE == MODE_DB) {
@@ -845,6 +857,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) __attribute__((amdgpu
         }
         apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
     }
+    const int slot = rowmajor_slot(p.g, t.tid);
     if (KIND == MK_QCRIT) {
         R qv = 0;
         R* su = (R*)(sfl + p.su_offset);
@@ -854,7 +867,10 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) __attribute__((amdgpu
             const int pos[3] = {t.x, t.y, t.z};
             const int nn[3] = {p.g.nx, p.g.ny, p.g.nz};
             const int ls[3] = {1, t.hx, t.hxy};
-            auto grad = [&](int comp, int axis) { return fd_gradient<R, D>(pos, nn, su, sfl, t.center, ls, comp, axis); };
+            auto grad = [&](int comp, int axis) {
+                return boundary ? fd_gradient<R, D, true>(pos, nn, su, sfl, t.center, ls, comp, axis)
+                                : fd_gradient<R, D, false>(pos, nn, su, sfl, t.center, ls, comp, axis);
+            };
             R wsq;
             if (D == 2) {
                 const R w = grad(1, 0) - grad(0, 1);
@@ -877,9 +893,9 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) __attribute__((amdgpu
             pi_neq<L, R>(fn, pi);
             qv = (R)0.25 * wsq - (R)0.5 * strain_norm_sq<L, R>(pi, rho, p.tau);
         }
-        __shared__ R qbuf[MaxThreads<Q>::value];
-        const R qa[1] = {qv};
-        store_rowmajor<R, 1>(p.g, t.bx, t.by, t.bz, t.tid, qbuf, qa, p.qout, false);
+        stage[slot] = qv;
+        __syncthreads();
+        write_rowmajor<R, 1>(p.g, t.bx, t.by, t.bz, t.tid, stage, p.qout, false);
         return;
     }
     constexpr bool REDUCE = KIND == MK_REDUCE;
@@ -900,13 +916,15 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) __attribute__((amdgpu
             for (int d = 0; d < D; ++d) u[d] = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, d)];
             rho = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, D)];
         }
-        __shared__ R obuf[MaxThreads<Q>::value * D];
-        const R ra[1] = {rho};
-        store_rowmajor<R, 1>(p.g, t.bx, t.by, t.bz, t.tid, obuf, ra, p.rho, false);
-        R ua[D];
+        // rho and u staged together: one barrier
+        const bool planar = p.u_layout != LBM_U_XYZD;
+        const int tc = p.g.tile_cells;
+        stage[slot] = rho;
 #pragma unroll
-        for (int d = 0; d < D; ++d) ua[d] = u[d];
-        store_rowmajor<R, D>(p.g, t.bx, t.by, t.bz, t.tid, obuf, ua, p.u, p.u_layout != LBM_U_XYZD);
+        for (int d = 0; d < D; ++d) stage[tc + (planar ? d * tc + slot : slot * D + d)] = u[d];
+        __syncthreads();
+        write_rowmajor<R, 1>(p.g, t.bx, t.by, t.bz, t.tid, stage, p.rho, false);
+        write_rowmajor<R, D>(p.g, t.bx, t.by, t.bz, t.tid, stage + tc, p.u, planar);
         return;
     }
     // Deterministic block reduction in FP64: wave partials by lane order, then waves in order.

@@ -72,9 +72,9 @@ template <class L, class S> struct LaunchImpl {
         }
     }
 
-    static int step(const Handle& h, void* f_out, const void* f_in, const void* bc, const uint8_t* flags, double tau,
-                    int mode, hipStream_t st) {
-        if ((uintptr_t)flags & 15u) return set_error(LBM_ERR_INVALID, "flags must be 16-byte aligned");
+    // The per-tile flag summary of (flags, bc): rebuilt unless static flags are promised and it was
+    // built from these buffers.
+    static int ensure_summary(const Handle& h, const uint8_t* flags, const void* bc, hipStream_t st) {
         const uint32_t n_chunks = (uint32_t)(((size_t)h.g.n_tiles * h.g.tile_cells) >> 4);
         if (!h.static_flags || h.summary_flags != (const void*)flags || h.summary_bc != bc) {
             hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, flags,
@@ -83,6 +83,13 @@ template <class L, class S> struct LaunchImpl {
             h.summary_flags = flags;
             h.summary_bc = bc;
         }
+        return LBM_OK;
+    }
+
+    static int step(const Handle& h, void* f_out, const void* f_in, const void* bc, const uint8_t* flags, double tau,
+                    int mode, hipStream_t st) {
+        if ((uintptr_t)flags & 15u) return set_error(LBM_ERR_INVALID, "flags must be 16-byte aligned");
+        if (int rc = ensure_summary(h, flags, bc, st)) return rc;
         StepParams<R> p;
         p.f_in = f_in;
         p.f_out = f_out;
@@ -107,9 +114,15 @@ template <class L, class S> struct LaunchImpl {
     }
 
     template <int MODE, bool DDF, int KIND>
-    static int macro_k(const Handle& h, const MacroParams<R>& p, hipStream_t st) {
-        // the Q-criterion also stages the velocity tile + halo behind the flag tile
-        const size_t lds = KIND == MK_QCRIT ? p.su_offset + h.lds_bytes * L::D * sizeof(R) : h.lds_bytes;
+    static int macro_k(const Handle& h, MacroParams<R> p, hipStream_t st) {
+        // LDS: flag tile + halo, then (Q-criterion) the velocity tile + halo, then the output staging
+        p.su_offset = (uint32_t)((h.lds_bytes + 15) & ~(size_t)15);
+        const size_t su_bytes = KIND == MK_QCRIT ? h.lds_bytes * L::D * sizeof(R) : 0;
+        p.stage_offset = (uint32_t)((p.su_offset + su_bytes + 15) & ~(size_t)15);
+        const size_t stage_bytes = KIND == MK_FIELDS ? (size_t)h.g.tile_cells * (L::D + 1) * sizeof(R)
+                                   : KIND == MK_QCRIT ? (size_t)h.g.tile_cells * sizeof(R) : 0;
+        const size_t lds = p.stage_offset + stage_bytes;
+        if (lds > 65536) return set_error(LBM_ERR_UNSUPPORTED, "tile too large for the output kernel's LDS (64 KiB)");
         if (h.big)
             hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, KIND, true>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
                                lds, st, p);
@@ -129,8 +142,14 @@ template <class L, class S> struct LaunchImpl {
     }
 
     static MacroParams<R> macro_params(const Handle& h, const void* f, const void* bc, const uint8_t* flags,
-                                       bool mode_is_db) {
+                                       bool mode_is_db, hipStream_t st, int* rc) {
         MacroParams<R> p{};
+        *rc = LBM_OK;
+        // interior tiles skip the flags when the summary can be built (16-byte aligned flags)
+        if (!((uintptr_t)flags & 15u)) {
+            *rc = ensure_summary(h, flags, bc, st);
+            p.summary = h.summary;
+        }
         p.f = f;
         p.bc = (const R*)bc;
         p.flags = flags;
@@ -148,7 +167,9 @@ template <class L, class S> struct LaunchImpl {
 
     static int macro(const Handle& h, void* rho, void* u, const void* f, const void* bc, const uint8_t* flags,
                      int mode, int u_layout, hipStream_t st) {
-        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
+        int prc = LBM_OK;
+        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB, st, &prc);
+        if (prc) return prc;
         p.rho = (R*)rho;
         p.u = (R*)u;
         p.u_layout = u_layout;
@@ -157,20 +178,21 @@ template <class L, class S> struct LaunchImpl {
 
     static int qcrit(const Handle& h, void* q, const void* u, const void* f, const void* bc, const uint8_t* flags,
                      double tau, int mode, int u_layout, hipStream_t st) {
-        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
+        int prc = LBM_OK;
+        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB, st, &prc);
+        if (prc) return prc;
         p.vel = (const R*)u;
         p.qout = (R*)q;
         p.tau = (R)tau;
         p.u_layout = u_layout;
-        p.su_offset = (uint32_t)((h.lds_bytes + 15) & ~(size_t)15);
-        if (p.su_offset + h.lds_bytes * L::D * sizeof(R) > 65536)
-            return set_error(LBM_ERR_UNSUPPORTED, "tile too large for the Q-criterion's LDS velocity tile (64 KiB)");
         return macro_any<MK_QCRIT>(h, p, mode, st);
     }
 
     static int reduce(const Handle& h, double* out, const void* f, const void* bc, const uint8_t* flags, int mode,
                       hipStream_t st) {
-        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB);
+        int prc = LBM_OK;
+        MacroParams<R> p = macro_params(h, f, bc, flags, mode == MODE_DB, st, &prc);
+        if (prc) return prc;
         p.partial = h.partial;
         int rc = macro_any<MK_REDUCE>(h, p, mode, st);
         if (rc) return rc;

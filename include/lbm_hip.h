@@ -213,6 +213,18 @@ int lbm_set_links_clamped(lbm_handle* h, const uint32_t* nodes, const float* qdi
 int lbm_link_force(lbm_handle* h, double* out_dev, void* stream);
 
 /*
+ * Per-node momentum-exchange force: while set, every lbm_step_db writes, for each link record r of
+ * the last lbm_set_links, node_force_dev[r*D + d] = the force of that node's links this step (the
+ * terms lbm_link_force sums), into caller-owned device memory of n_links * D doubles. Replaces the
+ * force_tensor[tid, d] output of calculate_drag_around_object / nodewise_bounceback_kernel
+ * (src/lbm/output/drag.mojo:104-105, src/lbm/geometry/nodewise_interpolated_BB.mojo:166-167); with
+ * the DDF shift the stored populations are shifted back (+ 2 w_q per link) so the force is the
+ * physical one. A record whose node is not a FLUID_SOLID_BOUNDARY node is not written. NULL stops
+ * the output; lbm_set_links clears it (the record count changes).
+ */
+int lbm_set_link_node_force(lbm_handle* h, double* node_force_dev);
+
+/*
  * Float16C codec on the device for n values: to_f16c = 1 converts floats to
  * uint16 codes, 0 converts codes to floats. The same codec the F16C storage
  * uses (Float16C.to_fp16c / to_fp32, src/utils/custom_fp.mojo:36-100), for

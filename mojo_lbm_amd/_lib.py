@@ -62,6 +62,7 @@ SIGNATURES = {
     "lbm_set_exterior_wall": ([_vp, _vp, _vp, _cp, _i, C.POINTER(_d), _d, _vp], _i),
     "lbm_set_links": ([_vp, _vp, _vp, C.c_uint32], _i),
     "lbm_set_links_clamped": ([_vp, _vp, _vp, C.c_uint32, C.c_float], _i),
+    "lbm_set_link_node_force": ([_vp, _vp], _i),
     "lbm_set_mlp": ([_vp, _vp, C.c_int32], _i),
     "lbm_mlp_forward": ([_vp, _vp, C.c_int64, _vp, _vp], _i),
     "lbm_mlp_closure": ([_vp, _vp, C.c_int64, _vp, _vp], _i),

@@ -365,6 +365,7 @@ int lbm_set_links_clamped(lbm_handle* hh, const uint32_t* nodes, const float* qd
     h.link_q = nullptr;
     h.n_links = 0;
     h.force_valid = false;
+    h.node_force = nullptr;  // sized for the old records
     if (!n_links) return LBM_OK;
     // q_dist = max(q_dist, q_clamp) on every link (qd >= 0); negative entries stay plain bounce-back
     std::vector<float> qd(qdist, qdist + (size_t)n_links * (h.d.Q - 1));
@@ -383,6 +384,14 @@ int lbm_set_links_clamped(lbm_handle* hh, const uint32_t* nodes, const float* qd
         hipMemcpy(h.link_q, qd.data(), qbytes, hipMemcpyHostToDevice) != hipSuccess)
         return set_error(LBM_ERR_HIP, "copy of the link records failed");
     h.n_links = n_links;
+    return LBM_OK;
+}
+
+int lbm_set_link_node_force(lbm_handle* hh, double* node_force_dev) {
+    if (int rc = check_handle(hh)) return rc;
+    Handle& h = *H(hh);
+    if (node_force_dev && !h.n_links) return set_error(LBM_ERR_INVALID, "set the links (lbm_set_links) first");
+    h.node_force = node_force_dev;
     return LBM_OK;
 }
 

@@ -38,6 +38,7 @@ template <class R> struct StepParams {
     const float* link_q;       // (Q-1) link distances per record
     uint32_t n_links;
     double* force_partial;     // per wave: D doubles of momentum-exchange force (when link_nodes != NULL)
+    double* node_force;        // per link record: D doubles of the node's force this step, or NULL
     const float* mlp;          // learned-closure parameter block (LES_MLP)
     Geo g;
 };
@@ -463,7 +464,8 @@ template <class L, class S, bool DDF, bool BIG>
 __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uint8_t* sfl, const Field<S, BIG>& f,
                                                   const uint32_t* __restrict__ link_nodes,
                                                   const float* __restrict__ link_q, uint32_t n_links, uint32_t node,
-                                                  typename S::R (&fv)[L::Q], double (&force)[3]) {
+                                                  typename S::R (&fv)[L::Q], double (&force)[3],
+                                                  double* __restrict__ node_force = nullptr) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
     uint32_t lo = 0, hi = n_links;
@@ -501,6 +503,12 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uin
         }
     }
     if (done) fv[0] = fv[0] - dm;
+    if (node_force) {
+        // this node's momentum exchange (force starts at zero for the node): the per-node force of
+        // calculate_drag_around_object, src/lbm/output/drag.mojo:104-105
+#pragma unroll
+        for (int d = 0; d < D; ++d) node_force[(size_t)lo * D + d] = force[d];
+    }
     return done;
 }
 
@@ -669,7 +677,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
                 uint32_t rebuilt = 0;
                 if (links && (t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY)
                     rebuilt = bouzidi_links<L, S, DDF, BIG>(t, sfl, fin, p.link_nodes, p.link_q, p.n_links,
-                                                            t.tile * t.tc + t.loc, fv, force);
+                                                            t.tile * t.tc + t.loc, fv, force, p.node_force);
                 apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, fin, p.bc, moving, p.eqbc, fv, rebuilt);
             }
         }

@@ -24,6 +24,7 @@ struct Handle {
     float* link_q = nullptr;
     uint32_t n_links = 0;
     double* force_partial = nullptr; // per wave of every tile: D doubles
+    double* node_force = nullptr;    // caller's per-link-record force output (lbm_set_link_node_force)
     float* mlp = nullptr;            // learned-closure parameter block (lbm_set_mlp)
     bool force_valid = false;        // a step with links has run since lbm_set_links
 };

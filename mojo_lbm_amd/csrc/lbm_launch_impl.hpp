@@ -90,7 +90,7 @@ template <class L, class S> struct LaunchImpl {
                     int mode, hipStream_t st) {
         if ((uintptr_t)flags & 15u) return set_error(LBM_ERR_INVALID, "flags must be 16-byte aligned");
         if (int rc = ensure_summary(h, flags, bc, st)) return rc;
-        StepParams<R> p;
+        StepParams<R> p{};
         p.f_in = f_in;
         p.f_out = f_out;
         p.bc = (const R*)bc;
@@ -106,6 +106,7 @@ template <class L, class S> struct LaunchImpl {
         p.link_q = h.link_q;
         p.n_links = h.n_links;
         p.force_partial = h.force_partial;
+        p.node_force = h.n_links ? h.node_force : nullptr;
         p.mlp = h.mlp;
         p.g = h.g;
         if (mode == MODE_DB) return step_mode<MODE_DB>(h, p, st);

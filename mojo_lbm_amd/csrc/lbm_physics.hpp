@@ -1,7 +1,8 @@
 // lbm_physics.hpp -- per-node LBM arithmetic (moments, f_eq, Smagorinsky, KBC,
 // SRT) in the reference's operation order. Shared by the kernels and by the
 // host-side initialisation (f_eq), and built with -ffp-contract=off so every
-// operation rounds as written.
+// operation rounds as written; f_eq and the SRT update use explicit fused
+// multiply-adds (fmadd), which the oracle evaluates the same way.
 #pragma once
 
 #include "lbm_common.hpp"
@@ -35,13 +36,20 @@ __host__ __device__ __forceinline__ R dot_uu(const R (&u)[3]) {
     return out;
 }
 
-// f_eq, src/lbm/kernels/utils/equilibrium.mojo:10-38.
+// a * b + c rounded once (an explicit fused multiply-add; the build otherwise contracts nothing).
+__host__ __device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__host__ __device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// f_eq, src/lbm/kernels/utils/equilibrium.mojo:10-38: w rho (1 + 3 eu + 4.5 eu^2 - 1.5 uu), or with the
+// DDF shift w rho (3 eu + 4.5 eu^2 - 1.5 uu) + w (rho - 1), evaluated with three fused multiply-adds:
+// A = fma(eu, fma(4.5, eu, 3), -(1.5 uu)), then fma(w rho, A, w rho) or fma(w rho, A, w (rho - 1)).
 template <class L, class R, bool DDF>
 __host__ __device__ __forceinline__ R feq(int q, R rho, const R (&u)[3], R uu) {
     const R eu = dot_c<L, R>(u, q);
     const R w = weight<L, R>(q);
-    if (DDF) return w * rho * ((R)3 * eu + (R)4.5 * eu * eu - (R)1.5 * uu) + w * (rho - (R)1);
-    return w * rho * ((R)1 + (R)3 * eu + (R)4.5 * eu * eu - (R)1.5 * uu);
+    const R A = fmadd(eu, fmadd((R)4.5, eu, (R)3), -((R)1.5 * uu));
+    const R wr = w * rho;
+    return DDF ? fmadd(wr, A, w * (rho - (R)1)) : fmadd(wr, A, wr);
 }
 
 // get_density (moment.mojo:17-41) and get_velocity (moment.mojo:45-76).
@@ -407,9 +415,10 @@ __device__ __forceinline__ void collide(R (&f)[L::Q], R tau, R inv_tau_base, R c
         }
         return;
     }
+    // SRT, ops/collisions/collision.mojo:12-46: f - (1/tau)(f - f_eq) as one fused multiply-add
     const R inv_tau = LES != LES_OFF ? (R)1 / tau_local : inv_tau_base;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = f[q] - inv_tau * (f[q] - feq<L, R, DDF>(q, rho, u, uu));
+    for (int q = 0; q < Q; ++q) f[q] = fmadd(-inv_tau, f[q] - feq<L, R, DDF>(q, rho, u, uu), f[q]);
 }
 
 }  // namespace lbm

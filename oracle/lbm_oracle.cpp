@@ -217,11 +217,15 @@ template <class T> struct Cell {
         return out;
     }
 
-    /* f_eq, src/lbm/kernels/utils/equilibrium.mojo:10-38. */
+    /* f_eq, src/lbm/kernels/utils/equilibrium.mojo:10-38: w rho (1 + 3 eu + 4.5 eu^2 - 1.5 uu), or
+     * w rho (3 eu + 4.5 eu^2 - 1.5 uu) + w (rho - 1) with the DDF shift, with the polynomial and the
+     * last multiply-add each rounded once (std::fma): A = fma(eu, fma(4.5, eu, 3), -(1.5 uu)),
+     * f_eq = fma(w rho, A, w rho) or fma(w rho, A, w (rho - 1)). */
     T feq(int q, T rho, const T* u, T uu, bool ddf) const {
         T eu = dot_c(u, q);
-        if (ddf) return w[q] * rho * ((T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * uu) + w[q] * (rho - (T)1);
-        return w[q] * rho * ((T)1 + (T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * uu);
+        const T A = std::fma(eu, std::fma((T)4.5, eu, (T)3), -((T)1.5 * uu));
+        const T wr = w[q] * rho;
+        return ddf ? std::fma(wr, A, w[q] * (rho - (T)1)) : std::fma(wr, A, wr);
     }
 
     /* get_density, src/lbm/kernels/utils/moment.mojo:17-41 (Vector.sum, vector.mojo:219-224). */
@@ -634,7 +638,7 @@ template <class S, class T> struct Stepper {
         // SRT
         T uu = cell.dot_uu(u);
         T inv_tau = (T)1 / tau_local;
-        for (int q = 0; q < L.Q; ++q) fv[q] -= inv_tau * (fv[q] - cell.feq(q, rho, u, uu, ddf));
+        for (int q = 0; q < L.Q; ++q) fv[q] = std::fma(-inv_tau, fv[q] - cell.feq(q, rho, u, uu, ddf), fv[q]);
     }
 
     bool valid(uint8_t flag) const { return !(flag_is_solid(flag) || flag_excluded(flag)); }  // is_valid_thread
@@ -666,7 +670,7 @@ float g_q_clamp = 0.0f;  // lower bound of every link fraction (q_clamp of the r
 template <class S, class T>
 int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uint8_t* flags, double tau,
               int mode, const uint32_t* link_nodes = nullptr, const float* link_q = nullptr, uint32_t n_links = 0,
-              double* force_out = nullptr) {
+              double* force_out = nullptr, double* node_force_out = nullptr) {
     Stepper<S, T> st(*d);
     st.link_nodes = link_nodes;
     st.link_q = link_q;
@@ -695,6 +699,18 @@ int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uin
         for (int k = 0; k < 3; ++k) force_out[k] = 0.0;
         for (long i = 0; i < ncell; ++i)
             for (int k = 0; k < 3; ++k) force_out[k] += nforce[(size_t)i * 3 + k];
+    }
+    if (force_out && node_force_out) {
+        // the force of each link record's node (drag.mojo:104-105), D values per record
+        for (long i = 0; i < ncell; ++i) {
+            int idx[3] = {(int)(i % g.nx), (int)((i / g.nx) % g.ny), (int)(i / ((long)g.nx * g.ny))};
+            const uint32_t node = (uint32_t)g.addr(idx[0], idx[1], idx[2], 0, 1);
+            const uint32_t* r = std::lower_bound(link_nodes, link_nodes + n_links, node);
+            if (r == link_nodes + n_links || *r != node) continue;
+            const uint8_t flag = flags[node];
+            if (!st.valid(flag) || (flag & 0x0F) != LBM_FLAG_FLUID_SOLID_BOUNDARY) continue;
+            for (int k = 0; k < st.L.D; ++k) node_force_out[(size_t)(r - link_nodes) * st.L.D + k] = nforce[(size_t)i * 3 + k];
+        }
     }
     return 0;
 }
@@ -1023,16 +1039,17 @@ int orc_step_db(const lbm_desc* d, void* f_out, const void* f_in, const void* bc
                  (step_impl<double, double>(d, (double*)f_out, (const double*)f_in, (const double*)bc, flags, tau, 0)))
 }
 
-/* Double-buffer step with interpolated bounce-back links; force_out[0..2] = total momentum-exchange force. */
+/* Double-buffer step with interpolated bounce-back links; force_out[0..2] = total momentum-exchange force,
+ * node_force_out (NULL or n_links * D doubles) = the force of each link record's node. */
 int orc_step_db_links(const lbm_desc* d, void* f_out, const void* f_in, const void* bc, const uint8_t* flags,
                       double tau, const uint32_t* link_nodes, const float* link_q, uint32_t n_links,
-                      double* force_out) {
+                      double* force_out, double* node_force_out) {
     ORC_DISPATCH((step_impl<float, float>(d, (float*)f_out, (const float*)f_in, (const float*)bc, flags, tau, 0,
-                                          link_nodes, link_q, n_links, force_out)),
+                                          link_nodes, link_q, n_links, force_out, node_force_out)),
                  (step_impl<uint16_t, float>(d, (uint16_t*)f_out, (const uint16_t*)f_in, (const float*)bc, flags, tau,
-                                             0, link_nodes, link_q, n_links, force_out)),
+                                             0, link_nodes, link_q, n_links, force_out, node_force_out)),
                  (step_impl<double, double>(d, (double*)f_out, (const double*)f_in, (const double*)bc, flags, tau, 0,
-                                            link_nodes, link_q, n_links, force_out)))
+                                            link_nodes, link_q, n_links, force_out, node_force_out)))
 }
 
 int orc_step_ep(const lbm_desc* d, void* f, const void* bc, const uint8_t* flags, double tau, int is_even) {

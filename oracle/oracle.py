@@ -45,7 +45,7 @@ def lib():
         L.orc_step_ep.argtypes = [pd, vp, vp, vp, d, i]
         L.orc_set_mlp.argtypes = [vp]
         L.orc_mlp_closure.argtypes = [vp, vp, C.c_long, vp]
-        L.orc_step_db_links.argtypes = [pd, vp, vp, vp, vp, d, vp, vp, C.c_uint32, vp]
+        L.orc_step_db_links.argtypes = [pd, vp, vp, vp, vp, d, vp, vp, C.c_uint32, vp, vp]
         L.orc_step_db_links.restype = i
         L.orc_macroscopic_links.argtypes = [pd, vp, vp, vp, vp, vp, i, vp, vp, C.c_uint32]
         L.orc_macroscopic_links.restype = i
@@ -156,13 +156,17 @@ def step_db(desc, f_out, f_in, bc, flags, tau):
     _ok(lib().orc_step_db(C.byref(desc), _p(f_out), _p(f_in), _p(bc), _p(flags), float(tau)), "step_db")
 
 
-def step_db_links(desc, f_out, f_in, bc, flags, tau, nodes, qdist):
-    """Double-buffer step with interpolated bounce-back links; returns the momentum-exchange force (D,)."""
+def step_db_links(desc, f_out, f_in, bc, flags, tau, nodes, qdist, node_force=None):
+    """Double-buffer step with interpolated bounce-back links; returns the momentum-exchange force (D,).
+    node_force (float64 array of len(nodes) * D, optional) receives the force of each record's node."""
     nodes = np.ascontiguousarray(nodes, np.uint32)
     qdist = np.ascontiguousarray(qdist, np.float32)
     force = np.zeros(3, np.float64)
+    if node_force is not None:
+        assert node_force.dtype == np.float64 and node_force.size == len(nodes) * desc.D
     _ok(lib().orc_step_db_links(C.byref(desc), _p(f_out), _p(f_in), _p(bc), _p(flags), float(tau), _p(nodes),
-                                _p(qdist), len(nodes), _p(force)), "step_db_links")
+                                _p(qdist), len(nodes), _p(force), None if node_force is None else _p(node_force)),
+        "step_db_links")
     return force[: desc.D]
 
 

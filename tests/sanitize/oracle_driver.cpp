@@ -12,7 +12,7 @@
 extern "C" {
 int orc_step_db(const lbm_desc*, void*, const void*, const void*, const uint8_t*, double);
 int orc_step_db_links(const lbm_desc*, void*, const void*, const void*, const uint8_t*, double, const uint32_t*,
-                      const float*, uint32_t, double*);
+                      const float*, uint32_t, double*, double*);
 int orc_step_ep(const lbm_desc*, void*, const void*, const uint8_t*, double, int);
 int orc_macroscopic(const lbm_desc*, void*, void*, const void*, const void*, const uint8_t*, int, int);
 int orc_reduce_moments(const lbm_desc*, double*, const void*, const void*, const uint8_t*, int);
@@ -67,8 +67,9 @@ static int run(lbm_desc d) {
             for (int k = 1; k < d.Q; ++k) qd.push_back(i % 2 ? 0.3f : 0.8f);
         for (uint32_t nd : nodes) flags[nd] = LBM_FLAG_FLUID_SOLID_BOUNDARY;
         double force[3] = {0, 0, 0};
+        std::vector<double> node_force(nodes.size() * (size_t)d.D);
         if (orc_step_db_links(&d, f2.data(), f.data(), bc.data(), flags.data(), 0.6, nodes.data(), qd.data(),
-                              (uint32_t)nodes.size(), force))
+                              (uint32_t)nodes.size(), force, node_force.data()))
             return fail("step_db_links");
         f.swap(f2);
     }

@@ -125,3 +125,39 @@ def test_links_porous_gaps_match_oracle(gpu, lat, storage, ddf, q_clamp):
     finally:
         O.set_q_clamp(0.0)
     assert np.allclose(run.link_force(), force, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("lat,storage,ddf", [("D2Q9", 0, 0), ("D3Q19", 1, 1), ("D3Q27", 2, 0)])
+def test_node_forces_match_oracle(gpu, lat, storage, ddf):
+    """lbm_set_link_node_force: the force of every link record's node after each step equals the
+    oracle's (the same FP64 sum in the same order), and the records sum to lbm_link_force."""
+    import torch
+
+    from mojo_lbm_amd import _lib
+
+    print(f"case node forces {lat} storage={storage} ddf={ddf}", flush=True)
+    desc, f, bc, flags, nodes, qdist = _porous_case(lat, storage, ddf, seed=11)
+    D = desc.D
+    run = U.DeviceRun(desc, f, bc, flags)
+    run.set_links(nodes, qdist)
+    nf = torch.full((len(nodes) * D,), float("nan"), dtype=torch.float64, device="cuda")
+    _lib.check(run.L.lbm_set_link_node_force(run.h.ptr, nf.data_ptr()))
+    a, b = f.copy(), f.copy()
+    onf = np.zeros(len(nodes) * D)
+    for step in range(2):
+        run.step_db(0.7)
+        O.step_db_links(desc, b, a, bc, flags, 0.7, nodes, qdist, node_force=onf)
+        a, b = b, a
+        torch.cuda.synchronize()
+        got = nf.cpu().numpy()
+        assert np.array_equal(got, onf), f"node forces differ after step {step + 1}"
+    assert np.allclose(got.reshape(-1, D).sum(axis=0), run.link_force()[:D], rtol=1e-12, atol=1e-12)
+    # new links clear the output; stopping it leaves the buffer alone
+    run.set_links(nodes, qdist)
+    nf.fill_(7.0)
+    run.step_db(0.7)
+    torch.cuda.synchronize()
+    assert torch.all(nf == 7.0)
+    _lib.check(run.L.lbm_set_links(run.h.ptr, None, None, 0))
+    with pytest.raises(_lib.LbmError):
+        _lib.check(run.L.lbm_set_link_node_force(run.h.ptr, nf.data_ptr()))

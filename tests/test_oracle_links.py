@@ -31,9 +31,11 @@ def _couette(qd, tau, ddf, steps, U_wall=0.05, n=16):
     qdist[:, c[1:, 1] == 1] = qd  # links towards the wall row y = 0
     flags[nodes] = 3
     a, b = f.copy(), f.copy()
+    node_force = np.zeros(len(nodes) * 2)
     for _ in range(steps):
-        force = O.step_db_links(desc, b, a, bc, flags, tau, nodes, qdist)
+        force = O.step_db_links(desc, b, a, bc, flags, tau, nodes, qdist, node_force=node_force)
         a, b = b, a
+    _couette.node_force = node_force.reshape(-1, 2)
     rho, u = O.macroscopic(desc, a, bc, flags, links=(nodes, qdist))
     y = np.arange(n)
     y_wall = 1.0 - float(np.float32(qd))
@@ -51,6 +53,18 @@ def test_bouzidi_couette_is_exact(qd, tau, ddf, steps):
     assert np.max(np.abs(ux - exact[None, :])) < 1e-10 * 0.05
     assert abs(force[0] - fx) < 1e-9 * fx
     assert abs(force[1] + fy) < 1e-9 * fy
+
+
+def test_node_forces_of_couette_flow():
+    """Per-node momentum exchange (drag.mojo:104-105): in steady Couette flow every one of the n wall
+    nodes carries the same force, nu U / H in x and -rho cs^2 in y per node, and the node forces sum
+    to the total."""
+    _, _, force, fx, fy = _couette(0.35, 0.8, 1, 8000)
+    nf = _couette.node_force
+    n = len(nf)
+    assert np.allclose(nf.sum(axis=0), force, rtol=1e-13, atol=0)
+    assert np.all(np.abs(nf[:, 0] - fx / n) < 1e-9 * fx / n)
+    assert np.all(np.abs(nf[:, 1] + fy / n) < 1e-9 * fy / n)
 
 
 def test_bouzidi_couette_detects_wrong_fraction():

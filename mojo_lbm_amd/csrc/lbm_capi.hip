@@ -70,8 +70,10 @@ int validate(const lbm_desc* d) {
     if (d->les == LBM_LES_MLP && d->D != 3)
         return set_error(LBM_ERR_UNSUPPORTED, "the learned closure needs a 3-D lattice");
     if (!(d->smagorinsky_cs >= 0.0)) return set_error(LBM_ERR_INVALID, "smagorinsky_cs must be >= 0");
-    const double elems = (double)d->nx * d->ny * d->nz * d->Q;
-    if (elems >= 4294967296.0) return set_error(LBM_ERR_UNSUPPORTED, "more than 2^32 populations per buffer");
+    // Node indices (flags, link records) are 32-bit; population and bc offsets are 64-bit in the
+    // kernels that buffers of 2 GiB and more use, so only the node count is bounded.
+    const double nodes = (double)d->nx * d->ny * d->nz;
+    if (nodes >= 4294967296.0) return set_error(LBM_ERR_UNSUPPORTED, "2^32 nodes or more");
     return LBM_OK;
 }
 

@@ -16,6 +16,8 @@
 // 2 GiB or more use a 64-bit pointer variant (BIG).
 #pragma once
 
+#include <type_traits>
+
 #include "lbm_physics.hpp"
 
 namespace lbm {
@@ -148,25 +150,30 @@ template <> __device__ __forceinline__ void register_fence<27>(int32_t (&r)[27])
 }
 #undef LBM_RF
 
+// Element index type of f and bc: 32 bits below 2 GiB per buffer, 64 bits above (BIG), so a
+// buffer may hold more than 2^32 values.
+template <bool BIG> using Idx = typename std::conditional<BIG, uint64_t, uint32_t>::type;
+
 template <class S, bool BIG> struct Field {
     using T = typename S::T;
     using R = typename S::R;
+    using I = Idx<BIG>;
     T* ptr;
     __amdgpu_buffer_rsrc_t rsrc;
     __device__ __forceinline__ Field(const void* p, uint32_t bytes) : ptr((T*)p) {
         if (!BIG) rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, (int)bytes, 0x00020000);
     }
-    __device__ __forceinline__ R ld(uint32_t lane, uint32_t uni) const {
-        if (BIG) return S::ld(ptr + ((size_t)lane + uni));
-        return Bits<S>::get(rsrc, lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T));
+    __device__ __forceinline__ R ld(I lane, uint32_t uni) const {
+        if (BIG) return S::ld(ptr + ((uint64_t)lane + uni));
+        return Bits<S>::get(rsrc, (uint32_t)lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T));
     }
     // Float16C, 32-bit addressing: the sign-extended code (decoded by f16c_decode_sext).
-    __device__ __forceinline__ int32_t ld_code(uint32_t lane, uint32_t uni) const {
-        return Bits<S>::raw(rsrc, lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T));
+    __device__ __forceinline__ int32_t ld_code(I lane, uint32_t uni) const {
+        return Bits<S>::raw(rsrc, (uint32_t)lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T));
     }
-    __device__ __forceinline__ void st(uint32_t lane, uint32_t uni, R v) const {
-        if (BIG) S::st(ptr + ((size_t)lane + uni), v);
-        else Bits<S>::put(rsrc, lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T), v);
+    __device__ __forceinline__ void st(I lane, uint32_t uni, R v) const {
+        if (BIG) S::st(ptr + ((uint64_t)lane + uni), v);
+        else Bits<S>::put(rsrc, (uint32_t)lane * (uint32_t)sizeof(T), uni * (uint32_t)sizeof(T), v);
     }
 };
 
@@ -176,7 +183,7 @@ template <class S, bool BIG> struct Field {
 // for boundary tiles, the LDS flag tile with its one-node halo. Halo flags are
 // fetched into registers before the population loads are issued and written
 // to LDS after, so their latency overlaps the population traffic.
-template <class L> struct TileCtx {
+template <class L, class I = uint32_t> struct TileCtx {
     static constexpr int MAXR = 8;  // halo rounds per lane (hcells <= 8 * tile_cells, checked on the host)
     int x, y, z;          // node
     int tid;
@@ -186,8 +193,8 @@ template <class L> struct TileCtx {
     int center;           // own position in the LDS tile (the workgroup's slab + halo)
     uint32_t tile;        // tile id
     int bx, by, bz;       // tile coordinates (uniform)
-    uint32_t fbase;       // element index of (x, q = 0) in f
-    uint32_t fm[3], fp[3];// element deltas of a -1 / +1 move in f (mod 2^32)
+    I fbase;              // element index of (x, q = 0) in f
+    I fm[3], fp[3];       // element deltas of a -1 / +1 move in f (modulo the index type)
     uint32_t tc;          // tile_cells: element distance between q planes
     int nonfluid;         // tile or halo holds a non-FLUID flag
     uint8_t own;
@@ -216,8 +223,8 @@ template <class L> struct TileCtx {
         hxy = hx * (ty + 2);
         center = (wlz + (D == 3 ? 1 : 0)) * hxy + (ly + 1) * hx + (lx + 1);
         tc = (uint32_t)g.tile_cells;
-        const uint32_t TQ = tc * (uint32_t)Q;
-        fbase = tile * TQ + loc;
+        const I TQ = (I)tc * (I)Q;
+        fbase = (I)tile * TQ + (I)loc;
 
         const int Ts[3] = {1, g.Tx, g.Tx * g.Ty};
         const int ls[3] = {1, tx, tx * ty};
@@ -230,16 +237,16 @@ template <class L> struct TileCtx {
             int dl, dt;
             if (lc[d] > 0) { dl = -ls[d]; dt = 0; }
             else { dl = (tn[d] - 1) * ls[d]; dt = (bc3[d] > 0 ? -1 : Tn[d] - 1) * Ts[d]; }
-            fm[d] = (uint32_t)dl + (uint32_t)dt * TQ;
+            fm[d] = (I)(int64_t)dl + (I)(int64_t)dt * TQ;
             if (lc[d] < tn[d] - 1) { dl = ls[d]; dt = 0; }
             else { dl = -(tn[d] - 1) * ls[d]; dt = (bc3[d] < Tn[d] - 1 ? 1 : -(Tn[d] - 1)) * Ts[d]; }
-            fp[d] = (uint32_t)dl + (uint32_t)dt * TQ;
+            fp[d] = (I)(int64_t)dl + (I)(int64_t)dt * TQ;
         }
     }
 
     // Element index in f (q plane excluded) of node x + s * c_q.
-    __device__ __forceinline__ uint32_t nb(int q, int s) const {
-        uint32_t a = fbase;
+    __device__ __forceinline__ I nb(int q, int s) const {
+        I a = fbase;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int c = s * L::c(q, d);
@@ -250,15 +257,15 @@ template <class L> struct TileCtx {
     }
 
     // Element index in a field with K values per node of node (xn, yn, zn), value k.
-    __device__ __forceinline__ uint32_t field_index(const Geo& g, int xn, int yn, int zn, int K, int k) const {
+    __device__ __forceinline__ I field_index(const Geo& g, int xn, int yn, int zn, int K, int k) const {
         const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
         const uint32_t loc = (uint32_t)((xn & (tx - 1)) + ((yn & (ty - 1)) << g.txb) + ((zn & (tz - 1)) << (g.txb + g.tyb)));
         const uint32_t tl = (uint32_t)((xn >> g.txb) + g.Tx * ((yn >> g.tyb) + g.Ty * (zn >> g.tzb)));
-        return (tl * (uint32_t)K + (uint32_t)k) * tc + loc;
+        return ((I)tl * (I)K + (I)k) * (I)tc + (I)loc;
     }
 
     // Element index of the pull neighbour x - c_q in a field with K values per node.
-    __device__ __forceinline__ uint32_t pull_index(const Geo& g, int q, int K, int k) const {
+    __device__ __forceinline__ I pull_index(const Geo& g, int q, int K, int k) const {
         int xn = x - L::c(q, 0), yn = y - L::c(q, 1), zn = z - L::c(q, 2);
         xn += xn < 0 ? g.nx : 0;
         xn -= xn >= g.nx ? g.nx : 0;
@@ -296,7 +303,7 @@ template <class L> struct TileCtx {
                     gz += gz < 0 ? g.nz : 0;
                     gz -= gz >= g.nz ? g.nz : 0;
                 }
-                hv[r] = flags[field_index(g, gx, gy, gz, 1, 0)];
+                hv[r] = flags[(uint32_t)field_index(g, gx, gy, gz, 1, 0)];
             }
         }
     }
@@ -366,8 +373,8 @@ __device__ __forceinline__ uint32_t boundary_tile(const Geo& g, const uint32_t* 
 // ops/load_and_store.mojo:164-221) or the esoteric slot loads
 // (esoteric_pull_load_f_vec, ops/load_and_store.mojo:11-86). Every address is
 // in bounds (periodic wrap), whatever the flags say.
-template <class L, int MODE, class LD>
-__device__ __forceinline__ void for_each_pull(const TileCtx<L>& t, LD&& ld) {
+template <class L, int MODE, class I, class LD>
+__device__ __forceinline__ void for_each_pull(const TileCtx<L, I>& t, LD&& ld) {
     constexpr int Q = L::Q;
     if (MODE == MODE_DB) {
 #pragma unroll
@@ -389,25 +396,25 @@ __device__ __forceinline__ void for_each_pull(const TileCtx<L>& t, LD&& ld) {
 }
 
 template <class L, class S, int MODE, bool BIG>
-__device__ __forceinline__ void load_populations(const TileCtx<L>& t, const Field<S, BIG>& f,
+__device__ __forceinline__ void load_populations(const TileCtx<L, Idx<BIG>>& t, const Field<S, BIG>& f,
                                                  typename S::R (&fv)[L::Q]) {
     constexpr int Q = L::Q;
     if constexpr (S::id == LBM_STORE_F16C && !BIG) {
         // all codes loaded sign-extended, then decoded: 3 instructions per population
         int32_t code[Q];
-        for_each_pull<L, MODE>(t, [&](int q, uint32_t lane, uint32_t uni) { code[q] = f.ld_code(lane, uni); });
+        for_each_pull<L, MODE>(t, [&](int q, Idx<BIG> lane, uint32_t uni) { code[q] = f.ld_code(lane, uni); });
         register_fence<Q>(code);
 #pragma unroll
         for (int q = 0; q < Q; ++q) fv[q] = f16c_decode_sext(code[q]);
     } else {
-        for_each_pull<L, MODE>(t, [&](int q, uint32_t lane, uint32_t uni) { fv[q] = f.ld(lane, uni); });
+        for_each_pull<L, MODE>(t, [&](int q, Idx<BIG> lane, uint32_t uni) { fv[q] = f.ld(lane, uni); });
     }
 }
 
 // Store: double buffer writes every population at x; esoteric_pull_store_f_vec
 // (ops/load_and_store.mojo:89-159) writes the parity's slot pattern.
 template <class L, class S, int MODE, bool BIG>
-__device__ __forceinline__ void store_populations(const TileCtx<L>& t, const Field<S, BIG>& f,
+__device__ __forceinline__ void store_populations(const TileCtx<L, Idx<BIG>>& t, const Field<S, BIG>& f,
                                                   const typename S::R (&fv)[L::Q]) {
     constexpr int Q = L::Q;
     if (MODE == MODE_DB) {
@@ -437,7 +444,7 @@ __device__ __forceinline__ void store_populations(const TileCtx<L>& t, const Fie
 // neighbour (raw flag compare, ops/load_and_store.mojo:216) is replaced by
 // f(x, opp q), loaded only for those directions.
 template <class L, class S, bool BIG>
-__device__ __forceinline__ void halfway_bounce_back(const TileCtx<L>& t, const uint8_t* sfl, const Field<S, BIG>& f,
+__device__ __forceinline__ void halfway_bounce_back(const TileCtx<L, Idx<BIG>>& t, const uint8_t* sfl, const Field<S, BIG>& f,
                                                     typename S::R (&fv)[L::Q]) {
 #pragma unroll
     for (int q = 1; q < L::Q; ++q)
@@ -461,7 +468,7 @@ __device__ __forceinline__ void halfway_bounce_back(const TileCtx<L>& t, const u
 // which adds dm instead; subtracting it is what conserves mass). Returns the rebuilt directions
 // (they take no moving-wall term).
 template <class L, class S, bool DDF, bool BIG>
-__device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uint8_t* sfl, const Field<S, BIG>& f,
+__device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L, Idx<BIG>>& t, const uint8_t* sfl, const Field<S, BIG>& f,
                                                   const uint32_t* __restrict__ link_nodes,
                                                   const float* __restrict__ link_q, uint32_t n_links, uint32_t node,
                                                   typename S::R (&fv)[L::Q], double (&force)[3],
@@ -513,7 +520,7 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L>& t, const uin
 }
 
 template <class L, class S, int MODE, bool DDF, bool BIG>
-__device__ __forceinline__ void apply_boundaries(const TileCtx<L>& t, const Geo& g, const uint8_t* sfl,
+__device__ __forceinline__ void apply_boundaries(const TileCtx<L, Idx<BIG>>& t, const Geo& g, const uint8_t* sfl,
                                                  const Field<S, BIG>& f, const typename S::R* __restrict__ bc,
                                                  int moving, int eqbc, typename S::R (&fv)[L::Q],
                                                  uint32_t rebuilt = 0) {
@@ -619,7 +626,7 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
 #pragma unroll
                 for (int d = 0; d <= D; ++d)
 #pragma unroll
-                    for (int j = 0; j < 16; ++j) b[d][j] = bc[(tile * (D + 1) + d) * tc + l0 + (uint32_t)j];
+                    for (int j = 0; j < 16; ++j) b[d][j] = bc[((size_t)tile * (D + 1) + d) * tc + l0 + (uint32_t)j];
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     bool mv = !isfinite(b[D][j]);
@@ -652,7 +659,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
                                           uint8_t* sfl, uint32_t part, int lzb) {
     using R = typename S::R;
     constexpr int Q = L::Q;
-    TileCtx<L> t;
+    TileCtx<L, Idx<BIG>> t;
     t.init_pos(p.g, tile, part, lzb);
     const uint32_t kind = p.summary ? boundary_tile<L::D>(p.g, p.summary, t.bx, t.by, t.bz) : 3u;
     const bool boundary = (kind & 1u) != 0;
@@ -840,7 +847,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
     extern __shared__ uint8_t sfl[];
     __shared__ double red[D + 1][16];
     R* stage = (R*)(sfl + p.stage_offset);  // output staging: FIELDS rho plane + u, QCRIT Q
-    TileCtx<L> t;
+    TileCtx<L, Idx<BIG>> t;
     t.init_pos(p.g, macro_tile(p.g), 0, p.g.tzb);
     const Field<S, BIG> f(p.f, p.f_bytes);
     const bool boundary = p.summary ? (boundary_tile<L::D>(p.g, p.summary, t.bx, t.by, t.bz) & 1u) != 0u : true;
@@ -1017,10 +1024,10 @@ __global__ void __launch_bounds__(256) wall_kernel(uint8_t* flags, R* bc, Geo g,
     idx[a2] = i / n[a1];
     const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
     const uint32_t loc = (uint32_t)((idx[0] & (tx - 1)) + ((idx[1] & (ty - 1)) << g.txb) + ((idx[2] & (tz - 1)) << (g.txb + g.tyb)));
-    const uint32_t tl = (uint32_t)((idx[0] >> g.txb) + g.Tx * ((idx[1] >> g.tyb) + g.Ty * (idx[2] >> g.tzb)));
-    flags[tl * (uint32_t)g.tile_cells + loc] = flag;
+    const size_t tl = (size_t)((idx[0] >> g.txb) + g.Tx * ((idx[1] >> g.tyb) + g.Ty * (idx[2] >> g.tzb)));
+    flags[tl * (size_t)g.tile_cells + loc] = flag;
     const R vals[4] = {u0, u1, u2, rho};
-    for (int k = 0; k <= D; ++k) bc[tl * (uint32_t)g.tile_cells * (uint32_t)(D + 1) + (uint32_t)k * g.tile_cells + loc] = k < D ? vals[k] : rho;
+    for (int k = 0; k <= D; ++k) bc[(tl * (size_t)(D + 1) + (size_t)k) * (size_t)g.tile_cells + loc] = k < D ? vals[k] : rho;
 }
 
 }  // namespace lbm

@@ -81,14 +81,14 @@ class Cavity:
     """A lid-driven cube on one handle: cavity nodes lo..n-1 per axis (walls at lo and n-1,
     SOLID below lo), lid at y = n-1 with u_x = u_lid (1 + amp xi), D3Q19 double buffer."""
 
-    def __init__(self, n, lo, tau, storage=0, collision="SRT", les=0, seed=0, amp=0.1, mlp=None, tile=8):
+    def __init__(self, n, lo, tau, storage=0, collision="SRT", les=0, seed=0, amp=0.1, mlp=None, tile=8, cs=None):
         torch = _torch()
         self.n, self.lo, self.tau = n, lo, tau
         ddf = 1 if storage == _lib.STORE_F16C else 0
         coll = {"SRT": _lib.COLL_SRT, "KBC": _lib.COLL_KBC}[collision]
         self.desc = _lib.LbmDesc(D=3, Q=19, nx=n, ny=n, nz=n, tx=tile, ty=tile, tz=tile, method=0, collision=coll,
                                  storage=storage, ddf_shift=ddf, les=les, include_moving_boundary=1,
-                                 included_bcs=3, smagorinsky_cs=CS)
+                                 included_bcs=3, smagorinsky_cs=CS if cs is None else cs)
         self.L = _lib.lib()
         self.h = _lib.Handle(self.desc)
         if mlp is not None:
@@ -175,8 +175,13 @@ def _finite(t) -> bool:
     return bool(_torch().isfinite(t).all().item())
 
 
-def generate(seeds, spinup, snapshots, interval, per_snapshot, collision="SRT", log=print):
-    """Training samples (X [n, 9], y [n]) on the host from fine runs with the given lid seeds."""
+def generate(seeds, spinup, snapshots, interval, per_snapshot, collision="SRT", log=print, on_policy=0):
+    """Training samples (X [n, 9], y [n]) on the host from fine runs with the given lid seeds.
+    on_policy > 0 adds, at every snapshot, samples whose features come from a coarse run
+    (Smagorinsky, Cs = 0.1) started from the filtered fine field and advanced alongside it (one
+    coarse step per four fine steps), restarted from the filtered field every on_policy
+    snapshots while it still tracks the fine run; the targets are the filtered field's. Their
+    features span what collide sees in coarse runs, which the filtered field alone does not."""
     rng = np.random.default_rng(12345)
     Xs, ys, info = [], [], []
     for seed in seeds:
@@ -192,7 +197,12 @@ def generate(seeds, spinup, snapshots, interval, per_snapshot, collision="SRT", 
             if not _finite(c):
                 raise FloatingPointError(f"fine run seed {seed} diverged by step {fine.steps_done}")
             log(f"# fine seed {seed}: {fine.steps_done} steps, {time.time() - t0:.0f} s")
+        coarse = None
         for s in range(snapshots):
+            if on_policy and s % on_policy == 0:
+                ref = filtered(fine).view(N_COARSE, N_COARSE, N_COARSE, 10)
+                coarse = Cavity(N_COARSE, 0, tau_for(N_COARSE), storage=_lib.STORE_F32, les=1, seed=seed)
+                coarse.set_fields(ref[..., 0].contiguous().view(-1), ref[..., 1:4].contiguous().view(-1))
             fine.step(interval)
             c = filtered(fine)
             cells = rng.integers(0, N_COARSE ** 3, size=per_snapshot).astype(np.uint32)
@@ -201,7 +211,17 @@ def generate(seeds, spinup, snapshots, interval, per_snapshot, collision="SRT", 
             keep = (yh >= 0) & np.isfinite(yh) & np.isfinite(Xh).all(axis=1)
             Xs.append(Xh[keep])
             ys.append(yh[keep])
-        info.append({"seed": seed, "steps": fine.steps_done, "seconds": round(time.time() - t0, 1)})
+            if coarse is not None:
+                coarse.step(interval // FACTOR)
+                Xc, fluid = closure_features(coarse)
+                ci = _torch().from_numpy(cells.astype(np.int64)).cuda()
+                Xo = Xc[ci].cpu().numpy()
+                ok = keep & fluid[ci].cpu().numpy() & np.isfinite(Xo).all(axis=1)
+                Xs.append(Xo[ok])
+                ys.append(yh[ok])
+        info.append({"seed": seed, "steps": fine.steps_done, "seconds": round(time.time() - t0, 1),
+                     "on_policy_restart": on_policy})
+        del coarse
         log(f"# fine seed {seed}: {snapshots} snapshots, {sum(len(a) for a in ys)} samples so far")
         del fine
         _torch().cuda.empty_cache()
@@ -284,6 +304,68 @@ def train(X, y, steps=6000, batch=16384, lr=3e-3, seed=0, log=print):
     return P, skill
 
 
+def closure_features(c: Cavity):
+    """The nine features collide computes for the learned closure at every node of a coarse
+    cavity's current state (FP32 storage): the populations are pulled from x - c_q (halfway
+    bounce-back from SOLID neighbours plus the moving-wall term 6 w rho_w (c . u_w)), then
+    Pi_neq (xx, xy, xz, yy, yz, zz), rho, |u| and the Smagorinsky nu_t (Cs = 0.1) of that
+    Pi_neq. Returns (features (n^3, 9) float32 in row-major node order, fluid mask (n^3,)),
+    both device tensors; rows of SOLID nodes are not meaningful."""
+    torch = _torch()
+    from .lattice import get_D3Q19
+
+    lat = get_D3Q19()
+    n, tile = c.n, c.desc.tx
+    lay = TiledLayouts(3, 19, (n, n, n), (tile, tile, tile))
+    idx = torch.from_numpy(lay.index_map(19)).cuda()
+    f = c.f.view(torch.float32)[idx]  # (n, n, n, 19) post-collision populations
+    bidx = torch.from_numpy(lay.index_map(4)).cuda()
+    bc = c.bc.view(torch.float32)[bidx]
+    solid = (c.flags[torch.from_numpy(lay.index_map(1)[..., 0]).cuda()] & 0x0F) == 1
+    cq = lat.directions
+    w = torch.tensor(lat.weights, dtype=torch.float32, device="cuda")
+    opp = lat.opposite_indices
+    fin = torch.empty_like(f)
+    for q in range(19):
+        sh = tuple(int(v) for v in cq[q])
+        pulled = torch.roll(f[..., q], shifts=sh, dims=(0, 1, 2))  # value at x - c_q
+        nb_solid = torch.roll(solid, shifts=sh, dims=(0, 1, 2))
+        nb_bc = torch.roll(bc, shifts=sh, dims=(0, 1, 2))
+        mw = 6.0 * w[q] * nb_bc[..., 3] * (nb_bc[..., 0] * sh[0] + nb_bc[..., 1] * sh[1] + nb_bc[..., 2] * sh[2])
+        fin[..., q] = torch.where(nb_solid, f[..., int(opp[q])] + (mw if q else 0.0), pulled)
+    fl = fin.reshape(-1, 19)
+    cqt = torch.tensor(cq, dtype=torch.float32, device="cuda")
+    rho = fl.sum(1)
+    u = fl @ cqt / rho[:, None]
+    uu = (u * u).sum(1)
+    eu = u @ cqt.T
+    feq = w[None, :] * rho[:, None] * (1 + 3 * eu + 4.5 * eu * eu - 1.5 * uu[:, None])
+    fn = fl - feq
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    pi = torch.stack([fn @ (cqt[:, a] * cqt[:, b]) for a, b in pairs], 1)
+    k = -1.0 / (2 * rho * (1.0 / 3.0) * c.tau)
+    S = pi * k[:, None]
+    snorm = (S * S * torch.tensor([1, 2, 2, 1, 2, 1], dtype=torch.float32, device="cuda")).sum(1)
+    nu_s = CS * CS * torch.sqrt(2 * snorm)
+    X = torch.cat([pi, rho[:, None], torch.sqrt(uu)[:, None], nu_s[:, None]], 1).contiguous()
+    return X, ~solid.reshape(-1)
+
+
+def fallback_fraction(c: Cavity, params) -> dict:
+    """Share of fluid nodes whose features lie outside the closure's training range (where
+    collide uses the Smagorinsky value), per feature and in total, and of 64-node waves in
+    which every node falls back."""
+    torch = _torch()
+    X, fluid = closure_features(c)
+    X = X[fluid]
+    P = torch.from_numpy(np.asarray(params, np.float32)).cuda()
+    lo, hi = P[1427:1436], P[1436:1445]
+    out = (X < lo) | (X > hi)
+    names = ["pi_xx", "pi_xy", "pi_xz", "pi_yy", "pi_yz", "pi_zz", "rho", "u", "nu_smag"]
+    return {"nodes": int(X.shape[0]), "fallback": float(out.any(1).float().mean()),
+            "per_feature": {nm: float(out[:, i].float().mean()) for i, nm in enumerate(names)}}
+
+
 def _coarse_u(c: Cavity):
     rho, u = c.macroscopic()
     return u.view(N_COARSE, N_COARSE, N_COARSE, 3)[1:-1, 1:-1, 1:-1].double()
@@ -297,13 +379,15 @@ def scaled_params(params, factor):
 
 
 def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=print, stats_steps=0,
-             sample_every=25, nu_scales=()):
+             sample_every=25, nu_scales=(), cs_values=()):
     """A-posteriori comparison of coarse closures against the filtered held-out fine run: velocity
     rel-L2 at checkpoints, then (stats_steps > 0) the rel-L2 of the time-mean velocity and of the
     rms velocity fluctuation over stats_steps further coarse steps sampled every sample_every steps
     (instantaneous fields decorrelate from the fine run within about a thousand coarse steps; the
     statistics do not). Each factor of nu_scales adds a coarse run of the closure with its nu_t
-    scaled by that factor ("learned_x<factor>")."""
+    scaled by that factor ("learned_x<factor>"), each value of cs_values a Smagorinsky run with
+    that constant ("smagorinsky_cs<value>"). At every checkpoint the share of the learned run's
+    nodes that fall back to Smagorinsky is recorded."""
     torch = _torch()
     t0 = time.time()
     fine = Cavity(N_FINE, FINE_LO, tau_for(N_FINE), storage=_lib.STORE_F16C, collision=collision, seed=seed)
@@ -321,9 +405,13 @@ def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=
     models = {"none": (0, None), "smagorinsky": (1, None), "learned": (2, params)}
     for fct in nu_scales:
         models[f"learned_x{fct:g}"] = (2, scaled_params(params, fct))
+    for cs in cs_values:
+        models[f"smagorinsky_cs{cs:g}"] = (1, None, cs)
     coarse = {}
-    for name, (les, p) in models.items():
-        c = Cavity(N_COARSE, 0, tau_for(N_COARSE), storage=_lib.STORE_F32, les=les, seed=seed, mlp=p)
+    for name, spec in models.items():
+        les, p = spec[0], spec[1]
+        c = Cavity(N_COARSE, 0, tau_for(N_COARSE), storage=_lib.STORE_F32, les=les, seed=seed, mlp=p,
+                   cs=spec[2] if len(spec) > 2 else CS)
         c.set_fields(rho0, u0)
         coarse[name] = c
     per = horizon // checkpoints
@@ -341,6 +429,7 @@ def evaluate(params, seed, spinup, horizon, checkpoints=4, collision="SRT", log=
             sums[name] = u.clone() if sums[name] is None else sums[name] + u
             err = float(torch.linalg.vector_norm(u - ref) / torch.linalg.vector_norm(ref))
             row[name] = err if math.isfinite(err) else None
+        row["learned_fallback"] = fallback_fraction(coarse["learned"], params)
         rows.append(row)
         log(f"# eval checkpoint {row}")
     # time-averaged velocity over the checkpoints
@@ -407,6 +496,8 @@ def main(argv=None):
     t.add_argument("--steps", type=int, default=6000)
     t.add_argument("--collision", default="SRT")
     t.add_argument("--coarse", type=int, default=128)
+    t.add_argument("--on-policy", type=int, default=0,
+                   help="add coarse-run features, restarting the coarse run every N snapshots (0: off)")
     t.add_argument("--out", required=True)
     e = sub.add_parser("evaluate")
     e.add_argument("--params", required=True)
@@ -418,6 +509,7 @@ def main(argv=None):
     e.add_argument("--stats-steps", type=int, default=0, help="coarse steps of time statistics after the checkpoints")
     e.add_argument("--sample-every", type=int, default=25)
     e.add_argument("--nu-scales", default="", help="comma-separated factors: extra closure runs with nu_t scaled")
+    e.add_argument("--cs-values", default="", help="comma-separated Smagorinsky constants: extra Smagorinsky runs")
     e.add_argument("--out", required=True)
     sc = sub.add_parser("scale", help="write a copy of a parameter file with the nu_t output scaled")
     sc.add_argument("--params", required=True)
@@ -434,20 +526,22 @@ def main(argv=None):
     if a.cmd == "train":
         seeds = [int(s) for s in a.seeds.split(",")]
         try:
-            X, y, info = generate(seeds, a.spinup, a.snapshots, a.interval, a.per_snapshot, a.collision, log)
+            X, y, info = generate(seeds, a.spinup, a.snapshots, a.interval, a.per_snapshot, a.collision, log,
+                                  a.on_policy)
         except FloatingPointError as exc:
             if a.collision != "SRT":
                 raise
             log(f"# {exc}; the fine runs are repeated with KBC collision")
             a.collision = "KBC"
-            X, y, info = generate(seeds, a.spinup, a.snapshots, a.interval, a.per_snapshot, a.collision, log)
+            X, y, info = generate(seeds, a.spinup, a.snapshots, a.interval, a.per_snapshot, a.collision, log,
+                                  a.on_policy)
         log(f"# {len(y)} samples; target mean {y.mean():.3e}, max {y.max():.3e}")
         P, skill = train(X, y, a.steps, log=log)
         out = {"params": [float(v) for v in P], "skill": skill, "runs": info,
                "config": {"fine": N_FINE, "coarse": N_COARSE, "u_lid": U_LID, "re": RE, "cs": CS,
                           "tau_fine": tau_for(N_FINE), "tau_coarse": tau_for(N_COARSE), "spinup": a.spinup,
                           "snapshots": a.snapshots, "interval": a.interval, "per_snapshot": a.per_snapshot,
-                          "collision": a.collision}}
+                          "collision": a.collision, "on_policy": a.on_policy}}
         with open(a.out, "w") as fh:
             json.dump(out, fh)
         log(json.dumps({k: v for k, v in skill.items() if k != "val_loss_history"}))
@@ -462,8 +556,9 @@ def main(argv=None):
         P = np.asarray(saved["params"], np.float32)
         coll = saved.get("config", {}).get("collision", a.collision)
         scales = [float(v) for v in a.nu_scales.split(",") if v.strip()]
+        css = [float(v) for v in a.cs_values.split(",") if v.strip()]
         res = evaluate(P, a.seed, a.spinup, a.horizon, collision=coll, log=log, stats_steps=a.stats_steps,
-                       sample_every=a.sample_every, nu_scales=scales)
+                       sample_every=a.sample_every, nu_scales=scales, cs_values=css)
         res["fine_collision"] = coll
         with open(a.out, "w") as fh:
             json.dump(res, fh)

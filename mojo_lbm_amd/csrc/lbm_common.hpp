@@ -86,29 +86,33 @@ __device__ __forceinline__ float f16c_decode_sext(int32_t hs) {
 }
 __device__ __forceinline__ float f16c_decode(uint32_t h) { return f16c_decode_sext((int16_t)(uint16_t)h); }
 
-// Encode, for |x| = a (fp32 bits without the sign):
-//   normal range: adding half a code ulp (2^11 in fp32 bits) and truncating 12
-//   bits rounds half away from zero, and subtracting 112 << 11 rebiases the
-//   exponent: (a + 2^11 - (112 << 23)) >> 12, taken signed, is negative below
-//   2^-15 and at most the subnormal code up to 2^-14;
-//   subnormal range: with |x| clamped to 2^-14, u = |x| 2^25 is exact and at
-//   most 2048 (the first normal code), and the code rounds it half up:
-//   floor(u + 1/2), which v_cvt_rpi_i32_f32 computes without rounding the sum
-//   (the same as (floor(2u) + 1) >> 1).
-// The code is the larger of the two, clamped to 0x7FFF, which also saturates
-// magnitudes of 2 - 2^-12 and above, infinities and NaN. Checked against the
-// format definition on all 2^32 inputs.
-__device__ __forceinline__ uint32_t f16c_encode(float x) {
-    const uint32_t bits = __float_as_uint(x);
-    const uint32_t a = bits & 0x7FFFFFFFu;
-    const int32_t normal = (int32_t)(a - 0x37FFF800u) >> 12;
-    const float u = __uint_as_float(min(a, 0x38800000u)) * 0x1p25f;
+// Encode. The code is computed from u = |x| 2^25 (one multiply with the |.| operand
+// modifier, exact for finite x below 2^103, +inf above; NaN stays NaN):
+//   normal range: for a = the fp32 bits of |x|, adding half a code ulp (2^11 in fp32
+//   bits), truncating 12 bits and rebiasing the exponent by 112 gives the code, which
+//   rounds half away from zero; u's bits are a + (25 << 23), so
+//   n = (bits(u) - 0x447FF800) >> 12 (signed, no overflow for any u >= 0) is that code,
+//   negative below 2^-15 and at most the subnormal code up to 2^-14; for u = inf or NaN
+//   it is above 0x7FFF;
+//   subnormal range: the code rounds u half up, floor(u + 1/2), which v_cvt_rpi_i32_f32
+//   computes without rounding the sum (0 for NaN, saturated for u >= 2^31); clamped to
+//   2048, the first normal code, it never exceeds the normal code above 2^-14.
+// The code is the larger of the two; v_cvt_pk_i16_i32 saturates it to 0x7FFF (which
+// saturates magnitudes of 2 - 2^-12 and above, infinities and NaN) and places it in the
+// upper half, where one v_and_or_b32 adds the sign. Eight instructions; the result's upper
+// half is the code (stored with a d16_hi store). Checked against the format definition on
+// all 2^32 inputs.
+__device__ __forceinline__ uint32_t f16c_encode_hi(float x) {
+    const float u = __builtin_fabsf(x) * 0x1p25f;
     int32_t sub;
     __asm__("v_cvt_rpi_i32_f32 %0, %1" : "=v"(sub) : "v"(u));
-    int32_t code;  // min(max(normal, sub), 0x7FFF) in one instruction (sub <= 0x7FFF)
-    __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(code) : "v"(normal), "v"(sub), "s"(0x7FFF));
-    return ((bits >> 16) & 0x8000u) | (uint32_t)code;
+    const int32_t n = (int32_t)(__float_as_uint(u) - 0x447FF800u) >> 12;
+    const int32_t code = max(n, min(sub, 2048));
+    uint32_t pk;  // upper half: min(code, 0x7FFF) (code >= 0)
+    __asm__("v_cvt_pk_i16_i32 %0, %1, %2" : "=v"(pk) : "v"(code), "v"(code));
+    return (__float_as_uint(x) & 0x80000000u) | pk;
 }
+__device__ __forceinline__ uint32_t f16c_encode(float x) { return f16c_encode_hi(x) >> 16; }
 
 // ----------------------------------------------------------------- storage
 // Populations are stored as S::T and computed in S::R (FP32 for F32 and F16C).
@@ -125,7 +129,7 @@ struct StoreF16C {
     static constexpr int id = LBM_STORE_F16C;
     __device__ __forceinline__ static R ld(const T* p) { return f16c_decode(*p); }
     __device__ __forceinline__ static void st(T* p, R v) {
-        __builtin_nontemporal_store((uint16_t)f16c_encode(v), p);
+        __builtin_nontemporal_store((uint16_t)(f16c_encode_hi(v) >> 16), p);  // global_store_short_d16_hi
     }
 };
 struct StoreF64 {

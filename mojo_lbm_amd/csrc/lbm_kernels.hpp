@@ -113,8 +113,12 @@ template <> struct Bits<StoreF16C> {
     __device__ __forceinline__ static int32_t raw(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
         return (int16_t)__builtin_amdgcn_raw_buffer_load_b16(r, v, s, 0);
     }
+    // The encoder leaves the code in a register's upper half; the buffer-store builtin takes
+    // only a lower half (a shift per population), so the d16_hi store is written out.
     __device__ __forceinline__ static void put(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x) {
-        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)f16c_encode(x), r, v, s, kStoreNT);
+        __asm__ volatile("buffer_store_short_d16_hi %0, %1, %2, %3 offen nt"
+                         :
+                         : "v"(f16c_encode_hi(x)), "v"(v), "s"(r), "s"(s));
     }
 };
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -232,15 +236,17 @@ template <class L, class I = uint32_t> struct TileCtx {
         const int bc3[3] = {bx, by, bz};
         const int tn[3] = {tx, ty, tz};
         const int Tn[3] = {g.Tx, g.Ty, g.Tz};
+        // A move stays inside the tile unless the lane sits on the tile's face; the face case
+        // (into the neighbour tile, periodic) depends only on the tile, so it is computed once
+        // per workgroup and each lane selects.
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            int dl, dt;
-            if (lc[d] > 0) { dl = -ls[d]; dt = 0; }
-            else { dl = (tn[d] - 1) * ls[d]; dt = (bc3[d] > 0 ? -1 : Tn[d] - 1) * Ts[d]; }
-            fm[d] = (I)(int64_t)dl + (I)(int64_t)dt * TQ;
-            if (lc[d] < tn[d] - 1) { dl = ls[d]; dt = 0; }
-            else { dl = -(tn[d] - 1) * ls[d]; dt = (bc3[d] < Tn[d] - 1 ? 1 : -(Tn[d] - 1)) * Ts[d]; }
-            fp[d] = (I)(int64_t)dl + (I)(int64_t)dt * TQ;
+            const I m_face = (I)(int64_t)((tn[d] - 1) * ls[d]) +
+                             (I)(int64_t)((bc3[d] > 0 ? -1 : Tn[d] - 1) * Ts[d]) * TQ;
+            const I p_face = (I)(int64_t)(-(tn[d] - 1) * ls[d]) +
+                             (I)(int64_t)((bc3[d] < Tn[d] - 1 ? 1 : -(Tn[d] - 1)) * Ts[d]) * TQ;
+            fm[d] = lc[d] > 0 ? (I)(int64_t)(-ls[d]) : m_face;
+            fp[d] = lc[d] < tn[d] - 1 ? (I)(int64_t)ls[d] : p_face;
         }
     }
 

@@ -51,6 +51,8 @@ struct Batch {
     void* bc = nullptr;
     uint8_t* flags = nullptr;
     uint32_t* summary = nullptr;
+    uint32_t* tile_kind = nullptr;  // per batch tile: boundary / moving-wall bits
+    uint32_t* nbmask = nullptr;     // per node of the batch: neighbour masks of boundary tiles
     uint32_t* link_nodes = nullptr;
     float* link_q = nullptr;
     size_t link_cap = 0;
@@ -71,7 +73,7 @@ struct Batch {
 // instantiated in lbm_batch_inst_*.hip.
 template <class L, class S> struct BatchLaunch {
     static int step(const Batch& b, int parity, hipStream_t st);
-    static int prepare(Batch& b, hipStream_t st);  // per-case flag summaries + device case records
+    static int prepare(Batch& b, hipStream_t st);  // per-case flag summaries, masks + device case records
     static size_t rec_bytes();
     static int init(const Batch& b, hipStream_t st);
     static int force(const Batch& b, double* out, hipStream_t st);

@@ -18,7 +18,6 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
     batch_step_kernel(const BatchCaseRec<typename S::R>* __restrict__ cases, const uint32_t* __restrict__ tile_case,
                       uint32_t n_tiles, uint32_t chunk, int parity) {
     using R = typename S::R;
-    extern __shared__ uint8_t sfl[];
     const uint32_t n_slots = 8u * chunk;
     for (uint32_t b = blockIdx.x; b < n_slots; b += gridDim.x) {
         // slot b runs on XCD b % 8 (round-robin dispatch); XCD k walks tiles [k chunk, (k+1) chunk)
@@ -31,10 +30,8 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
                 p.f_in = p.f_out;
                 p.f_out = const_cast<void*>(a);
             }
-            step_tile<L, S, MODE_DB, COLL, LES, DDF, false, LINKS>(p, gt - cases[c].tile_begin, gt, sfl, 0,
-                                                                    p.g.tzb);
+            step_tile<L, S, MODE_DB, COLL, LES, DDF, false, LINKS>(p, gt - cases[c].tile_begin, gt, 0, p.g.tzb);
         }
-        if (b + gridDim.x < n_slots) __syncthreads();  // the next tile reuses the LDS flag tile
     }
 }
 
@@ -47,13 +44,11 @@ template <class L, class S> struct BatchLaunchImpl {
         const uint32_t slots = 8u * b.chunk;
         const uint32_t grid = b.grid_blocks > 0 ? std::min<uint32_t>((uint32_t)b.grid_blocks, slots) : slots;
         if (b.any_links)
-            hipLaunchKernelGGL((batch_step_kernel<L, S, COLL, LES, DDF, true>), dim3(grid), dim3(b.tile_cells),
-                               b.lds_bytes, st, (const BatchCaseRec<R>*)b.recs, b.tile_case, b.n_tiles, b.chunk,
-                               parity);
+            hipLaunchKernelGGL((batch_step_kernel<L, S, COLL, LES, DDF, true>), dim3(grid), dim3(b.tile_cells), 0, st,
+                               (const BatchCaseRec<R>*)b.recs, b.tile_case, b.n_tiles, b.chunk, parity);
         else
-            hipLaunchKernelGGL((batch_step_kernel<L, S, COLL, LES, DDF, false>), dim3(grid), dim3(b.tile_cells),
-                               b.lds_bytes, st, (const BatchCaseRec<R>*)b.recs, b.tile_case, b.n_tiles, b.chunk,
-                               parity);
+            hipLaunchKernelGGL((batch_step_kernel<L, S, COLL, LES, DDF, false>), dim3(grid), dim3(b.tile_cells), 0, st,
+                               (const BatchCaseRec<R>*)b.recs, b.tile_case, b.n_tiles, b.chunk, parity);
         return check_launch("batch_step_kernel");
     }
 
@@ -87,12 +82,18 @@ template <class L, class S> struct BatchLaunchImpl {
             hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, fl, bcp,
                                (int)(b.proto.include_moving_boundary != 0), sm, g, n_chunks);
             if (int rc = check_launch("tile_summary_kernel")) return rc;
+            uint32_t* tk = b.tile_kind + b.tile_begin[c];
+            uint32_t* nm = b.nbmask + (size_t)b.tile_begin[c] * b.tile_cells;
+            hipLaunchKernelGGL((node_mask_kernel<L>), dim3(g.n_tiles), dim3(g.tile_cells), b.lds_bytes, st, fl, sm, tk,
+                               nm, g);
+            if (int rc = check_launch("node_mask_kernel")) return rc;
             StepParams<R> p{};
             p.f_in = (const T*)b.f[0] + b.f_off[c];
             p.f_out = (T*)b.f[1] + b.f_off[c];
             p.bc = bcp;
             p.flags = fl;
-            p.summary = sm;
+            p.tile_kind = tk;
+            p.nbmask = nm;
             p.tau = (R)b.cases[c].tau;
             p.inv_tau = (R)1 / p.tau;
             p.cs = (R)b.proto.smagorinsky_cs;

@@ -200,6 +200,15 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out) {
         delete h;
         return set_error(LBM_ERR_HIP, "hipMalloc of the tile summary failed");
     }
+    // tile kinds and the per-node neighbour masks (4 bytes per node; read for boundary tiles only)
+    if (hipMalloc((void**)&h->tile_kind, (size_t)h->g.n_tiles * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&h->nbmask, (size_t)h->g.n_tiles * h->g.tile_cells * sizeof(uint32_t)) != hipSuccess) {
+        if (h->tile_kind) (void)hipFree(h->tile_kind);
+        (void)hipFree(h->summary);
+        (void)hipFree(h->partial);
+        delete h;
+        return set_error(LBM_ERR_HIP, "hipMalloc of the neighbour masks failed");
+    }
     *out = reinterpret_cast<lbm_handle*>(h);
     return LBM_OK;
 }
@@ -209,6 +218,8 @@ int lbm_destroy(lbm_handle* hh) {
     Handle* h = H(hh);
     if (h->partial) (void)hipFree(h->partial);
     if (h->summary) (void)hipFree(h->summary);
+    if (h->tile_kind) (void)hipFree(h->tile_kind);
+    if (h->nbmask) (void)hipFree(h->nbmask);
     if (h->link_nodes) (void)hipFree(h->link_nodes);
     if (h->link_q) (void)hipFree(h->link_q);
     if (h->force_partial) (void)hipFree(h->force_partial);

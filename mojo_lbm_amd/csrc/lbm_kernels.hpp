@@ -4,11 +4,12 @@
 // column-major index inside the tile, so every population plane of a tile is
 // read and written as contiguous 64-lane runs.
 //
-// A pre-pass (tile_summary_kernel) ORs the flags of every tile. A tile whose
-// own and 26 neighbouring tiles hold only FLUID nodes is an interior tile: it
-// pulls every population from x - c_q and collides, with no flag traffic, no
-// LDS and no barrier. Other tiles stage their flags plus a one-node halo in
-// LDS and apply bounce-back, moving walls and equilibrium nodes.
+// Pre-passes, run once while the flags stay the same (or before every step): tile_summary_kernel
+// ORs the flags of every tile, node_mask_kernel marks each tile interior or boundary and gives
+// every node of a boundary tile a 32-bit neighbour mask. A tile whose own and 26 neighbouring
+// tiles hold only FLUID nodes is an interior tile: it pulls every population from x - c_q and
+// collides, with no flag traffic. Other tiles load one mask per node beside the populations and
+// apply bounce-back, moving walls and equilibrium nodes (no LDS, no barrier in the step).
 //
 // Populations are addressed through buffer resources: each lane carries a
 // 32-bit byte offset and the q-plane offset q * tile_cells rides in the scalar
@@ -29,7 +30,8 @@ template <class R> struct StepParams {
     void* f_out;               // == f_in for esoteric pull
     const R* bc;
     const uint8_t* flags;
-    const uint32_t* summary;   // per-tile OR of the flags; NULL: every tile takes the boundary path
+    const uint32_t* tile_kind; // per tile: bit 0 boundary tile, bit 1 next to a moving wall (node_mask_kernel)
+    const uint32_t* nbmask;    // per node of a boundary tile: its neighbour mask (node_mask_kernel)
     R tau;
     R inv_tau;                 // 1/tau rounded in R
     R cs;
@@ -90,6 +92,16 @@ __host__ __device__ constexpr uint32_t axis_regions(int d, int s) {
     return m;
 }
 
+// ------------------------------------------------------------ neighbour masks
+// Per-node mask of the nodes of boundary tiles, built from the flags by node_mask_kernel (once
+// while the flags stay the same): bit q (1 <= q < Q) is set when the pull neighbour x - c_q is
+// SOLID (raw flag compare, ops/load_and_store.mojo:216); the upper bits describe the node. A
+// mask of 0 is a FLUID node with no SOLID neighbour: nothing to do beyond the plain update.
+constexpr uint32_t NB_INVALID = 1u << 0;  // SOLID or EXCLUDE node (is_valid_thread false)
+constexpr uint32_t NB_EQ = 1u << 27;      // EQUILIBRIUM node
+constexpr uint32_t NB_FSB = 1u << 28;     // FLUID_SOLID_BOUNDARY node (interpolated links)
+constexpr uint32_t NB_DECOR = 1u << 29;   // a pull neighbour is SOLID with more flag bits set
+
 // ------------------------------------------------------------ population access
 // Cache policy of population stores: bit 1 (nt) streams them past the caches;
 // a step never rereads what it writes.
@@ -114,9 +126,13 @@ template <> struct Bits<StoreF16C> {
         return (int16_t)__builtin_amdgcn_raw_buffer_load_b16(r, v, s, 0);
     }
     // The encoder leaves the code in a register's upper half; the buffer-store builtin takes
-    // only a lower half (a shift per population), so the d16_hi store is written out.
+    // only a lower half (a shift per population), so the d16_hi store is written out. The
+    // compiler does not see the store's reads of its scalar operands, which it may have just
+    // reloaded with a VALU v_readlane (a spilled SGPR): a VALU write of an SGPR needs five wait
+    // states before a vector memory instruction reads it, so the store waits them out itself
+    // (the wave stalls; other waves issue meanwhile).
     __device__ __forceinline__ static void put(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x) {
-        __asm__ volatile("buffer_store_short_d16_hi %0, %1, %2, %3 offen nt"
+        __asm__ volatile("s_nop 4\n\tbuffer_store_short_d16_hi %0, %1, %2, %3 offen nt"
                          :
                          : "v"(f16c_encode_hi(x)), "v"(v), "s"(r), "s"(s));
     }
@@ -341,6 +357,21 @@ __device__ __forceinline__ bool node_valid(uint8_t fl) {
     return !(((fl & 0x0F) == LBM_FLAG_SOLID) || (fl & LBM_FLAG_EXCLUDE));  // is_valid_thread
 }
 
+// The node's neighbour mask (NB_* layout) from the LDS flag tile of a boundary tile.
+template <class L, class I>
+__device__ __forceinline__ uint32_t lds_mask(const TileCtx<L, I>& t, const uint8_t* sfl) {
+    uint32_t m = node_valid(t.own) ? 0u : NB_INVALID;
+    if ((t.own & 0x0F) == LBM_FLAG_EQUILIBRIUM) m |= NB_EQ;
+    if ((t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY) m |= NB_FSB;
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) {
+        const uint8_t nf = t.pull_flag(sfl, q);
+        if (nf == LBM_FLAG_SOLID) m |= 1u << q;
+        else if ((nf & 0x0F) == LBM_FLAG_SOLID) m |= NB_DECOR;
+    }
+    return m;
+}
+
 // Whether a non-FLUID node (bit 0) or a moving SOLID node (bit 1) lies in the
 // tile or in the one-node halo it reads from its neighbour tiles (periodic):
 // neighbour (dx, dy, dz) contributes its region (-dx, -dy, -dz). Uniform across
@@ -450,11 +481,11 @@ __device__ __forceinline__ void store_populations(const TileCtx<L, Idx<BIG>>& t,
 // neighbour (raw flag compare, ops/load_and_store.mojo:216) is replaced by
 // f(x, opp q), loaded only for those directions.
 template <class L, class S, bool BIG>
-__device__ __forceinline__ void halfway_bounce_back(const TileCtx<L, Idx<BIG>>& t, const uint8_t* sfl, const Field<S, BIG>& f,
+__device__ __forceinline__ void halfway_bounce_back(const TileCtx<L, Idx<BIG>>& t, uint32_t m, const Field<S, BIG>& f,
                                                     typename S::R (&fv)[L::Q]) {
 #pragma unroll
     for (int q = 1; q < L::Q; ++q)
-        if (t.pull_flag(sfl, q) == LBM_FLAG_SOLID) fv[q] = f.ld(t.fbase, (uint32_t)opp(q) * t.tc);
+        if ((m >> q) & 1u) fv[q] = f.ld(t.fbase, (uint32_t)opp(q) * t.tc);
 }
 
 // Interpolated bounce-back (linear Bouzidi, Firdaouss & Lallemand 2001; the formulas of
@@ -474,7 +505,7 @@ __device__ __forceinline__ void halfway_bounce_back(const TileCtx<L, Idx<BIG>>& 
 // which adds dm instead; subtracting it is what conserves mass). Returns the rebuilt directions
 // (they take no moving-wall term).
 template <class L, class S, bool DDF, bool BIG>
-__device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L, Idx<BIG>>& t, const uint8_t* sfl, const Field<S, BIG>& f,
+__device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L, Idx<BIG>>& t, uint32_t m, const Field<S, BIG>& f,
                                                   const uint32_t* __restrict__ link_nodes,
                                                   const float* __restrict__ link_q, uint32_t n_links, uint32_t node,
                                                   typename S::R (&fv)[L::Q], double (&force)[3],
@@ -494,7 +525,7 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L, Idx<BIG>>& t,
     for (int k = 1; k < Q; ++k) {
         const int q = opp(k);  // opp(1), 1, opp(3), 3, ...
         const R qd = (R)link_q[(size_t)lo * (Q - 1) + (q - 1)];
-        if (!(qd >= (R)0) || t.pull_flag(sfl, q) != LBM_FLAG_SOLID) continue;
+        if (!(qd >= (R)0) || !((m >> q) & 1u)) continue;
         const R a = fv[q];
         R fb;
         if (qd <= (R)0.5) {
@@ -526,17 +557,20 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L, Idx<BIG>>& t,
 }
 
 template <class L, class S, int MODE, bool DDF, bool BIG>
-__device__ __forceinline__ void apply_boundaries(const TileCtx<L, Idx<BIG>>& t, const Geo& g, const uint8_t* sfl,
-                                                 const Field<S, BIG>& f, const typename S::R* __restrict__ bc,
-                                                 int moving, int eqbc, typename S::R (&fv)[L::Q],
-                                                 uint32_t rebuilt = 0) {
+__device__ __forceinline__ void apply_boundaries(const TileCtx<L, Idx<BIG>>& t, const Geo& g, uint32_t m,
+                                                 const uint8_t* __restrict__ flags, const Field<S, BIG>& f,
+                                                 const typename S::R* __restrict__ bc, int moving, int eqbc,
+                                                 typename S::R (&fv)[L::Q], uint32_t rebuilt = 0) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
     if (moving) {
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
-            const uint8_t nf = t.pull_flag(sfl, q);
-            if ((nf & 0x0F) == LBM_FLAG_SOLID && !((rebuilt >> q) & 1u)) {
+            // a SOLID neighbour by its low flag bits: the mask bit, or (rarely: SOLID with more flag
+            // bits set somewhere around the node) the neighbour's flag itself
+            bool solid = (m >> q) & 1u;
+            if (!solid && (m & NB_DECOR)) solid = (flags[t.pull_index(g, q, 1, 0)] & 0x0F) == LBM_FLAG_SOLID;
+            if (solid && !((rebuilt >> q) & 1u)) {
                 R uw[3];
 #pragma unroll
                 for (int d = 0; d < D; ++d) uw[d] = bc[t.pull_index(g, q, D + 1, d)];
@@ -545,7 +579,7 @@ __device__ __forceinline__ void apply_boundaries(const TileCtx<L, Idx<BIG>>& t, 
             }
         }
     }
-    if (eqbc && (t.own & 0x0F) == LBM_FLAG_EQUILIBRIUM) {
+    if (eqbc && (m & NB_EQ)) {
         R ub[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) ub[d] = bc[t.field_index(g, t.x, t.y, t.z, D + 1, d)];
@@ -654,6 +688,26 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
     }
 }
 
+// The per-tile kind (bit 0: the tile or its one-node halo holds a non-FLUID node, bit 1: a
+// moving SOLID node) and, for boundary tiles, every node's neighbour mask: set_adjacent_flags
+// (src/lbm/kernels/utils/set_flags.mojo) done once per flag state instead of in every step. One
+// workgroup per tile (tile_cells lanes), the tile's flags and their one-node halo staged in LDS.
+template <class L>
+__global__ void __launch_bounds__(1024) node_mask_kernel(const uint8_t* __restrict__ flags,
+                                                         const uint32_t* __restrict__ summary,
+                                                         uint32_t* __restrict__ tile_kind,
+                                                         uint32_t* __restrict__ nbmask, Geo g) {
+    extern __shared__ uint8_t sfl[];
+    TileCtx<L, uint32_t> t;
+    t.init_pos(g, blockIdx.x, 0, g.tzb);
+    const uint32_t kind = boundary_tile<L::D>(g, summary, t.bx, t.by, t.bz);
+    if (threadIdx.x == 0) tile_kind[t.tile] = kind;
+    if (!(kind & 1u)) return;  // uniform
+    t.fetch_flags(g, flags);
+    t.commit_flags(g, sfl);
+    nbmask[(size_t)t.tile * t.tc + t.loc] = lds_mask(t, sfl);
+}
+
 // double_buffer_kernel (src/lbm/kernels/double_buffer.mojo:25-96) and
 // esoteric_pull_kernel[is_even] (src/lbm/kernels/esoteric_pull.mojo:24-111).
 //
@@ -662,37 +716,35 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
 // plain kernels, whose register budget it would otherwise set.
 template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS>
 __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, uint32_t tile, uint32_t force_slot,
-                                          uint8_t* sfl, uint32_t part, int lzb) {
+                                          uint32_t part, int lzb) {
     using R = typename S::R;
     constexpr int Q = L::Q;
     TileCtx<L, Idx<BIG>> t;
     t.init_pos(p.g, tile, part, lzb);
-    const uint32_t kind = p.summary ? boundary_tile<L::D>(p.g, p.summary, t.bx, t.by, t.bz) : 3u;
+    const uint32_t kind = p.tile_kind[t.tile];  // uniform
     const bool boundary = (kind & 1u) != 0;
     const int moving = p.moving && (kind & 2u);
     const Field<S, BIG> fin(p.f_in, p.f_bytes);
     const Field<S, BIG> fout(p.f_out, p.f_bytes);
-    if (boundary) t.fetch_flags(p.g, p.flags);
+    uint32_t m = 0;  // issued before the population loads: its latency overlaps theirs
+    if (boundary) m = p.nbmask[(size_t)t.tile * t.tc + t.loc];
     R fv[Q];
     load_populations<L, S, MODE, BIG>(t, fin, fv);
     constexpr bool links = LINKS && MODE == MODE_DB;
     double force[3] = {0.0, 0.0, 0.0};
     bool active = true;
-    if (boundary) {
-        t.commit_flags(p.g, sfl);
-        if (t.nonfluid) {
-            if (!node_valid(t.own)) {
-                // stays for the force reduction, or for the learned closure, which needs every lane
-                if (!links && LES != LES_MLP) return;
-                active = false;
-            } else {
-                if (MODE == MODE_DB) halfway_bounce_back<L, S, BIG>(t, sfl, fin, fv);
-                uint32_t rebuilt = 0;
-                if (links && (t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY)
-                    rebuilt = bouzidi_links<L, S, DDF, BIG>(t, sfl, fin, p.link_nodes, p.link_q, p.n_links,
-                                                            t.tile * t.tc + t.loc, fv, force, p.node_force);
-                apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, fin, p.bc, moving, p.eqbc, fv, rebuilt);
-            }
+    if (m) {
+        if (m & NB_INVALID) {
+            // stays for the force reduction, or for the learned closure, which needs every lane
+            if (!links && LES != LES_MLP) return;
+            active = false;
+        } else {
+            if (MODE == MODE_DB) halfway_bounce_back<L, S, BIG>(t, m, fin, fv);
+            uint32_t rebuilt = 0;
+            if (links && (m & NB_FSB))
+                rebuilt = bouzidi_links<L, S, DDF, BIG>(t, m, fin, p.link_nodes, p.link_q, p.n_links,
+                                                        t.tile * t.tc + t.loc, fv, force, p.node_force);
+            apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, fin, p.bc, moving, p.eqbc, fv, rebuilt);
         }
     }
     if (LES == LES_MLP) {
@@ -719,10 +771,9 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
 // tile-major, so the force records keep the whole-tile order (slot = unit, wave inside it).
 template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS>
 __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParams<typename S::R> p) {
-    extern __shared__ uint8_t sfl[];
     const int s = p.g.tzb - p.g.wzb;
     const uint32_t unit = (uint32_t)tile_of_block(p.g, s);
-    step_tile<L, S, MODE, COLL, LES, DDF, BIG, LINKS>(p, unit >> s, unit, sfl, unit & ((1u << s) - 1u), p.g.wzb);
+    step_tile<L, S, MODE, COLL, LES, DDF, BIG, LINKS>(p, unit >> s, unit, unit & ((1u << s) - 1u), p.g.wzb);
 }
 
 // The step with the learned closure, held to at most 128 VGPRs (two 512-lane workgroups per CU)
@@ -730,10 +781,9 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParam
 template <class L, class S, int MODE, int COLL, bool DDF>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8)))
     step_kernel_mlp(StepParams<typename S::R> p) {
-    extern __shared__ uint8_t sfl[];
     const int s = p.g.tzb - p.g.wzb;
     const uint32_t unit = (uint32_t)tile_of_block(p.g, s);
-    step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, unit >> s, unit, sfl, unit & ((1u << s) - 1u), p.g.wzb);
+    step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, unit >> s, unit, unit & ((1u << s) - 1u), p.g.wzb);
 }
 
 // get_velocity_gradient (src/lbm/kernels/utils/finite_difference.mojo:15-136), dx = 1: the
@@ -867,16 +917,17 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
         t.own = LBM_FLAG_FLUID;
     }
     if (t.nonfluid) {
+        const uint32_t m = lds_mask(t, sfl);
         uint32_t rebuilt = 0;
         if (MODE == MODE_DB) {
-            halfway_bounce_back<L, S, BIG>(t, sfl, f, fv);
-            if (p.link_nodes && (t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY) {
+            halfway_bounce_back<L, S, BIG>(t, m, f, fv);
+            if (p.link_nodes && (m & NB_FSB)) {
                 double unused[3] = {0, 0, 0};
-                rebuilt = bouzidi_links<L, S, DDF, BIG>(t, sfl, f, p.link_nodes, p.link_q, p.n_links,
+                rebuilt = bouzidi_links<L, S, DDF, BIG>(t, m, f, p.link_nodes, p.link_q, p.n_links,
                                                         t.tile * t.tc + t.loc, fv, unused);
             }
         }
-        apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, sfl, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
+        apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
     }
     const int slot = rowmajor_slot(p.g, t.tid);
     if (KIND == MK_QCRIT) {

@@ -17,6 +17,8 @@ struct Handle {
     bool big = false;            // f_bytes >= 2^31: 64-bit addressing
     double* partial = nullptr;   // per-tile reduction partials (n_tiles * (D+1))
     uint32_t* summary = nullptr; // per-tile region masks, rebuilt before every step
+    uint32_t* tile_kind = nullptr;  // per tile: boundary / moving-wall bits (node_mask_kernel)
+    uint32_t* nbmask = nullptr;     // per node: neighbour mask (read for boundary tiles only)
     bool static_flags = false;   // LBM_OPT_STATIC_FLAGS: rebuild only for new flags / bc buffers
     mutable const void* summary_flags = nullptr;  // buffers the summary was built from
     mutable const void* summary_bc = nullptr;

@@ -16,11 +16,9 @@ template <class L, class S> struct LaunchImpl {
         const int s = h.g.tzb - h.g.wzb;  // 2^s workgroups per tile, one z slab each
         const dim3 grid((unsigned)h.g.n_tiles << s), block((unsigned)h.g.tile_cells >> s);
         if (h.big)
-            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, true, LINKS>), grid, block, h.lds_bytes, st,
-                               p);
+            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, true, LINKS>), grid, block, 0, st, p);
         else
-            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, false, LINKS>), grid, block, h.lds_bytes, st,
-                               p);
+            hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, false, LINKS>), grid, block, 0, st, p);
         return check_launch("step_kernel");
     }
 
@@ -42,7 +40,7 @@ template <class L, class S> struct LaunchImpl {
                 return set_error(LBM_ERR_UNSUPPORTED, "the learned closure needs tiles of whole 64-lane waves");
             const int s = h.g.tzb - h.g.wzb;  // 2^s workgroups per tile, one z slab each
             hipLaunchKernelGGL((step_kernel_mlp<L, S, MODE, COLL, DDF>), dim3((unsigned)h.g.n_tiles << s),
-                               dim3((unsigned)h.g.tile_cells >> s), h.lds_bytes, st, p);
+                               dim3((unsigned)h.g.tile_cells >> s), 0, st, p);
             return check_launch("step_kernel");
         } else {
             return set_error(LBM_ERR_UNSUPPORTED, "the learned closure needs a 3-D lattice");
@@ -72,14 +70,18 @@ template <class L, class S> struct LaunchImpl {
         }
     }
 
-    // The per-tile flag summary of (flags, bc): rebuilt unless static flags are promised and it was
-    // built from these buffers.
+    // The per-tile flag summary of (flags, bc), the tile kinds and the neighbour masks of the
+    // boundary tiles: rebuilt unless static flags are promised and they were built from these
+    // buffers.
     static int ensure_summary(const Handle& h, const uint8_t* flags, const void* bc, hipStream_t st) {
         const uint32_t n_chunks = (uint32_t)(((size_t)h.g.n_tiles * h.g.tile_cells) >> 4);
         if (!h.static_flags || h.summary_flags != (const void*)flags || h.summary_bc != bc) {
             hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, flags,
                                (const R*)bc, (int)(h.d.include_moving_boundary != 0), h.summary, h.g, n_chunks);
             if (int rc = check_launch("tile_summary_kernel")) return rc;
+            hipLaunchKernelGGL((node_mask_kernel<L>), dim3(h.g.n_tiles), dim3(h.g.tile_cells), h.lds_bytes, st, flags,
+                               h.summary, h.tile_kind, h.nbmask, h.g);
+            if (int rc = check_launch("node_mask_kernel")) return rc;
             h.summary_flags = flags;
             h.summary_bc = bc;
         }
@@ -95,7 +97,8 @@ template <class L, class S> struct LaunchImpl {
         p.f_out = f_out;
         p.bc = (const R*)bc;
         p.flags = flags;
-        p.summary = h.summary;
+        p.tile_kind = h.tile_kind;
+        p.nbmask = h.nbmask;
         p.tau = (R)tau;
         p.inv_tau = (R)1 / p.tau;
         p.cs = (R)h.d.smagorinsky_cs;

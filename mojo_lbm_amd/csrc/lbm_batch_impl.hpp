@@ -84,8 +84,8 @@ template <class L, class S> struct BatchLaunchImpl {
             if (int rc = check_launch("tile_summary_kernel")) return rc;
             uint32_t* tk = b.tile_kind + b.tile_begin[c];
             uint32_t* nm = b.nbmask + (size_t)b.tile_begin[c] * b.tile_cells;
-            hipLaunchKernelGGL((node_mask_kernel<L>), dim3(g.n_tiles), dim3(g.tile_cells), b.lds_bytes, st, fl, sm, tk,
-                               nm, g);
+            hipLaunchKernelGGL((node_mask_kernel<L, R>), dim3(g.n_tiles), dim3(g.tile_cells), b.lds_bytes, st, fl, bcp,
+                               (int)(b.proto.include_moving_boundary != 0), sm, tk, nm, g);
             if (int rc = check_launch("node_mask_kernel")) return rc;
             StepParams<R> p{};
             p.f_in = (const T*)b.f[0] + b.f_off[c];

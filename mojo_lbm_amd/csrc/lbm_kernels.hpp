@@ -101,6 +101,7 @@ constexpr uint32_t NB_INVALID = 1u << 0;  // SOLID or EXCLUDE node (is_valid_thr
 constexpr uint32_t NB_EQ = 1u << 27;      // EQUILIBRIUM node
 constexpr uint32_t NB_FSB = 1u << 28;     // FLUID_SOLID_BOUNDARY node (interpolated links)
 constexpr uint32_t NB_DECOR = 1u << 29;   // a pull neighbour is SOLID with more flag bits set
+constexpr uint32_t NB_MOVING = 1u << 30;  // a SOLID pull neighbour is not a plain resting wall
 
 // ------------------------------------------------------------ population access
 // Cache policy of population stores: bit 1 (nt) streams them past the caches;
@@ -563,7 +564,7 @@ __device__ __forceinline__ void apply_boundaries(const TileCtx<L, Idx<BIG>>& t, 
                                                  typename S::R (&fv)[L::Q], uint32_t rebuilt = 0) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
-    if (moving) {
+    if (moving && (m & NB_MOVING)) {
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
             // a SOLID neighbour by its low flag bits: the mask bit, or (rarely: SOLID with more flag
@@ -692,20 +693,41 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
 // moving SOLID node) and, for boundary tiles, every node's neighbour mask: set_adjacent_flags
 // (src/lbm/kernels/utils/set_flags.mojo) done once per flag state instead of in every step. One
 // workgroup per tile (tile_cells lanes), the tile's flags and their one-node halo staged in LDS.
-template <class L>
+// With moving walls included, a node with a SOLID pull neighbour whose bc velocity is non-zero or
+// whose bc density is not finite is marked NB_MOVING, and so is a node with a decorated SOLID
+// neighbour (its flag is read again in the step). Next to any other SOLID node the moving-wall
+// term 6 w rho (c . u) is a signed zero, which the step does not add: it can change only the sign
+// of a zero population, and the collision that follows (f - f_eq, with f_eq != 0) does not carry
+// that sign into what it stores.
+template <class L, class R>
 __global__ void __launch_bounds__(1024) node_mask_kernel(const uint8_t* __restrict__ flags,
+                                                         const R* __restrict__ bc, int moving,
                                                          const uint32_t* __restrict__ summary,
                                                          uint32_t* __restrict__ tile_kind,
                                                          uint32_t* __restrict__ nbmask, Geo g) {
+    constexpr int D = L::D;
     extern __shared__ uint8_t sfl[];
-    TileCtx<L, uint32_t> t;
+    TileCtx<L, uint64_t> t;  // 64-bit bc indices
     t.init_pos(g, blockIdx.x, 0, g.tzb);
-    const uint32_t kind = boundary_tile<L::D>(g, summary, t.bx, t.by, t.bz);
+    const uint32_t kind = boundary_tile<D>(g, summary, t.bx, t.by, t.bz);
     if (threadIdx.x == 0) tile_kind[t.tile] = kind;
     if (!(kind & 1u)) return;  // uniform
     t.fetch_flags(g, flags);
     t.commit_flags(g, sfl);
-    nbmask[(size_t)t.tile * t.tc + t.loc] = lds_mask(t, sfl);
+    uint32_t m = lds_mask(t, sfl);
+    if (moving && !(m & NB_INVALID)) {
+        bool mv = (m & NB_DECOR) != 0;
+#pragma unroll
+        for (int q = 1; q < L::Q; ++q) {
+            if ((t.pull_flag(sfl, q) & 0x0F) != LBM_FLAG_SOLID) continue;
+            bool resting = isfinite(bc[t.pull_index(g, q, D + 1, D)]);
+#pragma unroll
+            for (int d = 0; d < D; ++d) resting = resting && bc[t.pull_index(g, q, D + 1, d)] == (R)0;
+            mv = mv || !resting;
+        }
+        if (mv) m |= NB_MOVING;
+    }
+    nbmask[(size_t)t.tile * t.tc + t.loc] = m;
 }
 
 // double_buffer_kernel (src/lbm/kernels/double_buffer.mojo:25-96) and
@@ -723,7 +745,6 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
     t.init_pos(p.g, tile, part, lzb);
     const uint32_t kind = p.tile_kind[t.tile];  // uniform
     const bool boundary = (kind & 1u) != 0;
-    const int moving = p.moving && (kind & 2u);
     const Field<S, BIG> fin(p.f_in, p.f_bytes);
     const Field<S, BIG> fout(p.f_out, p.f_bytes);
     uint32_t m = 0;  // issued before the population loads: its latency overlaps theirs
@@ -744,7 +765,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
             if (links && (m & NB_FSB))
                 rebuilt = bouzidi_links<L, S, DDF, BIG>(t, m, fin, p.link_nodes, p.link_q, p.n_links,
                                                         t.tile * t.tc + t.loc, fv, force, p.node_force);
-            apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, fin, p.bc, moving, p.eqbc, fv, rebuilt);
+            apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, fin, p.bc, p.moving, p.eqbc, fv, rebuilt);
         }
     }
     if (LES == LES_MLP) {
@@ -917,7 +938,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
         t.own = LBM_FLAG_FLUID;
     }
     if (t.nonfluid) {
-        const uint32_t m = lds_mask(t, sfl);
+        const uint32_t m = lds_mask(t, sfl) | NB_MOVING;  // the moving-wall term read for every SOLID neighbour
         uint32_t rebuilt = 0;
         if (MODE == MODE_DB) {
             halfway_bounce_back<L, S, BIG>(t, m, f, fv);

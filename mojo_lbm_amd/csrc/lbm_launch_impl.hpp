@@ -79,8 +79,9 @@ template <class L, class S> struct LaunchImpl {
             hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, flags,
                                (const R*)bc, (int)(h.d.include_moving_boundary != 0), h.summary, h.g, n_chunks);
             if (int rc = check_launch("tile_summary_kernel")) return rc;
-            hipLaunchKernelGGL((node_mask_kernel<L>), dim3(h.g.n_tiles), dim3(h.g.tile_cells), h.lds_bytes, st, flags,
-                               h.summary, h.tile_kind, h.nbmask, h.g);
+            hipLaunchKernelGGL((node_mask_kernel<L, R>), dim3(h.g.n_tiles), dim3(h.g.tile_cells), h.lds_bytes, st,
+                               flags, (const R*)bc, (int)(h.d.include_moving_boundary != 0), h.summary, h.tile_kind,
+                               h.nbmask, h.g);
             if (int rc = check_launch("node_mask_kernel")) return rc;
             h.summary_flags = flags;
             h.summary_bc = bc;

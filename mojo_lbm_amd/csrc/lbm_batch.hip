@@ -46,7 +46,7 @@ static void free_graph(Batch& b) {
 
 static void free_device(Batch& b) {
     free_graph(b);
-    void* ptrs[] = {b.f[0], b.f[1], b.bc, b.flags, b.summary, b.tile_kind, b.nbmask, b.link_nodes, b.link_q,
+    void* ptrs[] = {b.f[0], b.f[1], b.bc, b.flags, b.summary, b.tile_kind, b.nbmask, b.mvs, b.btiles, b.link_nodes, b.link_q,
                     b.force_partial, b.recs, b.tile_case};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -55,6 +55,8 @@ static void free_device(Batch& b) {
     b.summary = nullptr;
     b.tile_kind = nullptr;
     b.nbmask = nullptr;
+    b.mvs = nullptr;
+    b.btiles = nullptr;
     b.link_nodes = nullptr;
     b.link_q = nullptr;
     b.link_cap = 0;
@@ -134,9 +136,11 @@ static int layout(Batch& b) {
     });
     bool ok = hipMalloc(&b.f[0], f * b.f_elem_bytes) == hipSuccess && hipMalloc(&b.f[1], f * b.f_elem_bytes) == hipSuccess &&
               hipMalloc(&b.bc, bcn * b.r_bytes) == hipSuccess && hipMalloc((void**)&b.flags, fl) == hipSuccess &&
-              hipMalloc((void**)&b.summary, (size_t)tiles * 2 * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void**)&b.summary, (size_t)tiles * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc((void**)&b.tile_kind, (size_t)tiles * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc((void**)&b.nbmask, (size_t)tiles * b.tile_cells * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc((void**)&b.mvs, fl) == hipSuccess &&
+              hipMalloc((void**)&b.btiles, ((size_t)tiles + 1) * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc((void**)&b.force_partial, (size_t)tiles * waves * D * sizeof(double)) == hipSuccess &&
               hipMalloc(&b.recs, rec * cd.size()) == hipSuccess &&
               hipMalloc((void**)&b.tile_case, (size_t)tiles * sizeof(uint32_t)) == hipSuccess;

@@ -53,6 +53,8 @@ struct Batch {
     uint32_t* summary = nullptr;
     uint32_t* tile_kind = nullptr;  // per batch tile: boundary / moving-wall bits
     uint32_t* nbmask = nullptr;     // per node of the batch: neighbour masks of boundary tiles
+    uint8_t* mvs = nullptr;         // per node of the batch: moving SOLID nodes
+    uint32_t* btiles = nullptr;     // boundary tile ids of one case, then their count (staging)
     uint32_t* link_nodes = nullptr;
     float* link_q = nullptr;
     size_t link_cap = 0;

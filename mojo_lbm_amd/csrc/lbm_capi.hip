@@ -195,14 +195,18 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out) {
         delete h;
         return set_error(LBM_ERR_HIP, "hipMalloc of the reduction partials failed");
     }
-    if (hipMalloc((void**)&h->summary, (size_t)h->g.n_tiles * 2 * sizeof(uint32_t)) != hipSuccess) {
+    if (hipMalloc((void**)&h->summary, (size_t)h->g.n_tiles * sizeof(uint32_t)) != hipSuccess) {
         (void)hipFree(h->partial);
         delete h;
         return set_error(LBM_ERR_HIP, "hipMalloc of the tile summary failed");
     }
     // tile kinds and the per-node neighbour masks (4 bytes per node; read for boundary tiles only)
     if (hipMalloc((void**)&h->tile_kind, (size_t)h->g.n_tiles * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void**)&h->nbmask, (size_t)h->g.n_tiles * h->g.tile_cells * sizeof(uint32_t)) != hipSuccess) {
+        hipMalloc((void**)&h->nbmask, (size_t)h->g.n_tiles * h->g.tile_cells * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&h->mvs, (size_t)h->g.n_tiles * h->g.tile_cells) != hipSuccess ||
+        hipMalloc((void**)&h->btiles, ((size_t)h->g.n_tiles + 1) * sizeof(uint32_t)) != hipSuccess) {
+        if (h->mvs) (void)hipFree(h->mvs);
+        if (h->nbmask) (void)hipFree(h->nbmask);
         if (h->tile_kind) (void)hipFree(h->tile_kind);
         (void)hipFree(h->summary);
         (void)hipFree(h->partial);
@@ -220,6 +224,8 @@ int lbm_destroy(lbm_handle* hh) {
     if (h->summary) (void)hipFree(h->summary);
     if (h->tile_kind) (void)hipFree(h->tile_kind);
     if (h->nbmask) (void)hipFree(h->nbmask);
+    if (h->mvs) (void)hipFree(h->mvs);
+    if (h->btiles) (void)hipFree(h->btiles);
     if (h->link_nodes) (void)hipFree(h->link_nodes);
     if (h->link_q) (void)hipFree(h->link_q);
     if (h->force_partial) (void)hipFree(h->force_partial);

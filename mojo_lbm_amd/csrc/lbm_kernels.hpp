@@ -30,7 +30,7 @@ template <class R> struct StepParams {
     void* f_out;               // == f_in for esoteric pull
     const R* bc;
     const uint8_t* flags;
-    const uint32_t* tile_kind; // per tile: bit 0 boundary tile, bit 1 next to a moving wall (node_mask_kernel)
+    const uint32_t* tile_kind; // per tile: 1 for a boundary tile (node_mask_kernel)
     const uint32_t* nbmask;    // per node of a boundary tile: its neighbour mask (node_mask_kernel)
     R tau;
     R inv_tau;                 // 1/tau rounded in R
@@ -54,7 +54,7 @@ template <class R> struct MacroParams {
     R* rho;
     R* u;
     double* partial;           // reduction mode: per-tile (D+1) sums
-    const uint32_t* summary;   // per-tile region masks (interior tiles skip the flags), or NULL
+    const uint32_t* tile_kind; // per tile: 1 for a boundary tile (interior tiles skip the flags), or NULL
     const uint32_t* link_nodes;  // interpolated bounce-back records (double buffer), or null
     const float* link_q;
     uint32_t n_links;
@@ -299,8 +299,10 @@ template <class L, class I = uint32_t> struct TileCtx {
         return field_index(g, xn, yn, zn, K, k);
     }
 
-    // Loads the flags of the workgroup's slab and its halo into registers (no LDS traffic yet).
-    __device__ __forceinline__ void fetch_flags(const Geo& g, const uint8_t* __restrict__ flags) {
+    // Loads the flags of the workgroup's slab and its halo into registers (no LDS traffic yet),
+    // with mvs the moving-wall bytes of the same cells in bits 8-15.
+    __device__ __forceinline__ void fetch_flags(const Geo& g, const uint8_t* __restrict__ flags,
+                                                const uint8_t* __restrict__ mvs = nullptr) {
         constexpr int D = L::D;
         const int hcells = hxy * (D == 3 ? wtz + 2 : 1);
         const int nthr = blockDim.x;
@@ -326,13 +328,16 @@ template <class L, class I = uint32_t> struct TileCtx {
                     gz += gz < 0 ? g.nz : 0;
                     gz -= gz >= g.nz ? g.nz : 0;
                 }
-                hv[r] = flags[(uint32_t)field_index(g, gx, gy, gz, 1, 0)];
+                const uint32_t fi = (uint32_t)field_index(g, gx, gy, gz, 1, 0);
+                hv[r] = flags[fi];
+                if (mvs) hv[r] |= (uint32_t)mvs[fi] << 8;
             }
         }
     }
 
-    // Writes the fetched flags to LDS and synchronises the workgroup.
-    __device__ __forceinline__ void commit_flags(const Geo& g, uint8_t* sfl) {
+    // Writes the fetched flags (and, with smv, the moving-wall bits) to LDS and synchronises the
+    // workgroup.
+    __device__ __forceinline__ void commit_flags(const Geo& g, uint8_t* sfl, uint8_t* smv = nullptr) {
         constexpr int D = L::D;
         const int hcells = hxy * (D == 3 ? wtz + 2 : 1);
         const int nthr = blockDim.x;
@@ -341,8 +346,11 @@ template <class L, class I = uint32_t> struct TileCtx {
         for (int r = 0; r < MAXR; ++r) {
             const int h = tid + r * nthr;
             if (r * nthr >= hcells) break;
-            if (h < hcells) sfl[h] = (uint8_t)hv[r];
-            any |= hv[r];
+            if (h < hcells) {
+                sfl[h] = (uint8_t)hv[r];
+                if (smv) smv[h] = (uint8_t)(hv[r] >> 8);
+            }
+            any |= hv[r] & 0xFFu;
         }
         nonfluid = __syncthreads_or((int)any);
         own = sfl[center];
@@ -373,14 +381,13 @@ __device__ __forceinline__ uint32_t lds_mask(const TileCtx<L, I>& t, const uint8
     return m;
 }
 
-// Whether a non-FLUID node (bit 0) or a moving SOLID node (bit 1) lies in the
-// tile or in the one-node halo it reads from its neighbour tiles (periodic):
-// neighbour (dx, dy, dz) contributes its region (-dx, -dy, -dz). Uniform across
-// the workgroup.
+// 1 when a non-FLUID node lies in the tile or in the one-node halo it reads from its neighbour
+// tiles (periodic): neighbour (dx, dy, dz) contributes its region (-dx, -dy, -dz). Uniform
+// across the workgroup.
 template <int D>
 __device__ __forceinline__ uint32_t boundary_tile(const Geo& g, const uint32_t* __restrict__ summary, int bx, int by,
                                                   int bz) {
-    uint32_t any = 0, mv = 0;
+    uint32_t any = 0;
     const int zlo = D == 3 ? -1 : 0, zhi = D == 3 ? 1 : 0;
 #pragma unroll
     for (int dz = zlo; dz <= zhi; ++dz) {
@@ -398,12 +405,11 @@ __device__ __forceinline__ uint32_t boundary_tile(const Geo& g, const uint32_t* 
                 tx += tx < 0 ? g.Tx : 0;
                 tx -= tx >= g.Tx ? g.Tx : 0;
                 const uint32_t t = tx + g.Tx * (ty + g.Ty * tz);
-                any |= summary[2 * t] & region_bit(-dx, -dy, -dz);
-                mv |= summary[2 * t + 1] & region_bit(-dx, -dy, -dz);
+                any |= summary[t] & region_bit(-dx, -dy, -dz);
             }
         }
     }
-    return (any != 0 ? 1u : 0u) | (mv != 0 ? 2u : 0u);
+    return any != 0 ? 1u : 0u;
 }
 
 // Stream, part 1 (issued before the flags are known): the pull of every
@@ -621,28 +627,29 @@ __device__ __forceinline__ void apply_boundaries(const TileCtx<L, Idx<BIG>>& t, 
 // ------------------------------------------------------------ kernels
 template <int Q> struct MaxThreads { static constexpr int value = (Q == 9) ? 1024 : 512; };
 
-// Per-tile region masks of the flags: summary[2 t] marks regions holding a
-// non-FLUID node, summary[2 t + 1] regions holding a SOLID node whose bc
-// velocity is non-zero or whose bc density is not finite (a moving wall: for
-// any other solid the moving-wall term adds exactly +0). Each lane covers 16
-// consecutive nodes of one tile (tile_cells is a power of two in [32, 1024]);
-// a tile is tile_cells / 16 consecutive lanes of one wave, OR-reduced by
-// shuffles. n_chunks = cells / 16 lanes in total.
+// Per-tile region masks of the flags: summary[t] marks the regions of tile t that hold a
+// non-FLUID node. With moving walls included it also writes mvs, one byte per node: 1 for a
+// SOLID node whose bc velocity is non-zero or whose bc density is not finite (a moving wall).
+// Each lane covers 16 consecutive nodes of one tile (tile_cells is a power of two in [32, 1024]);
+// a tile is tile_cells / 16 consecutive lanes of one wave, OR-reduced by shuffles. n_chunks =
+// cells / 16 lanes in total.
 template <int D, class R>
 __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __restrict__ flags,
-                                                           const R* __restrict__ bc, int moving,
-                                                           uint32_t* __restrict__ summary, Geo g,
+                                                           const R* __restrict__ bc, uint8_t* __restrict__ mvs,
+                                                           uint32_t* __restrict__ summary,
+                                                           uint32_t* __restrict__ btiles, Geo g,
                                                            uint32_t n_chunks) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // 16-byte chunk
-    uint32_t mask = 0, mmask = 0;
+    if (i == 0) btiles[g.n_tiles] = 0;  // boundary-tile count of the tile_kind_kernel that follows
+    uint32_t mask = 0;
     if (i < n_chunks) {
         const uint4 v = reinterpret_cast<const uint4*>(flags)[i];
+        uint4 mv = make_uint4(0u, 0u, 0u, 0u);
         if ((v.x | v.y | v.z | v.w) != 0u) {
             const uint32_t w[4] = {v.x, v.y, v.z, v.w};
             const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
             const uint32_t l0 = (i << 4) & (uint32_t)(g.tile_cells - 1);
-            const uint32_t tile = (i << 4) / (uint32_t)g.tile_cells;
-            uint32_t regs[16], solid = 0;
+            uint32_t solid = 0;
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const uint32_t fb = (w[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
@@ -655,79 +662,105 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
                 const uint32_t mz = D == 2 ? axis_regions(2, 0)
                                            : (axis_regions(2, 0) | (lz == 0 ? axis_regions(2, -1) : 0u) |
                                               (lz == tz - 1 ? axis_regions(2, 1) : 0u));
-                regs[j] = fb != 0u ? (mx & my & mz) : 0u;
-                mask |= regs[j];
+                if (fb != 0u) mask |= mx & my & mz;
                 solid |= ((fb & 0x0Fu) == LBM_FLAG_SOLID ? 1u : 0u) << j;
             }
-            if (moving && solid) {
+            if (mvs && solid) {
                 // the bc of all 16 nodes, loaded together (independent loads, one wait), then
                 // tested for the SOLID ones
+                const uint32_t tile = (i << 4) / (uint32_t)g.tile_cells;
                 const uint32_t tc = (uint32_t)g.tile_cells;
                 R b[D + 1][16];
 #pragma unroll
                 for (int d = 0; d <= D; ++d)
 #pragma unroll
                     for (int j = 0; j < 16; ++j) b[d][j] = bc[((size_t)tile * (D + 1) + d) * tc + l0 + (uint32_t)j];
+                uint32_t mw[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
-                    bool mv = !isfinite(b[D][j]);
+                    bool m = !isfinite(b[D][j]);
 #pragma unroll
-                    for (int d = 0; d < D; ++d) mv = mv || b[d][j] != (R)0;
-                    if (((solid >> j) & 1u) && mv) mmask |= regs[j];
+                    for (int d = 0; d < D; ++d) m = m || b[d][j] != (R)0;
+                    if (((solid >> j) & 1u) && m) mw[j >> 2] |= 1u << ((j & 3) * 8);
                 }
+                mv = make_uint4(mw[0], mw[1], mw[2], mw[3]);
             }
         }
+        if (mvs) reinterpret_cast<uint4*>(mvs)[i] = mv;
     }
     const uint32_t lanes_per_tile = (uint32_t)g.tile_cells >> 4;  // 2 .. 64
-    for (uint32_t off = 1; off < lanes_per_tile; off <<= 1) {
-        mask |= (uint32_t)__shfl_xor((int)mask, (int)off, 64);
-        mmask |= (uint32_t)__shfl_xor((int)mmask, (int)off, 64);
-    }
-    if ((threadIdx.x & (lanes_per_tile - 1u)) == 0u && i < n_chunks) {
-        summary[2 * (i / lanes_per_tile)] = mask;
-        summary[2 * (i / lanes_per_tile) + 1] = mmask;
+    for (uint32_t off = 1; off < lanes_per_tile; off <<= 1) mask |= (uint32_t)__shfl_xor((int)mask, (int)off, 64);
+    if ((threadIdx.x & (lanes_per_tile - 1u)) == 0u && i < n_chunks) summary[i / lanes_per_tile] = mask;
+}
+
+// The per-tile kind (1: the tile or its one-node halo holds a non-FLUID node) and, for boundary
+// tiles, every node's neighbour mask: set_adjacent_flags
+// (src/lbm/kernels/utils/set_flags.mojo) done once per flag state instead of in every step. One
+// workgroup per tile (tile_cells lanes), the tile's flags and their one-node halo staged in LDS.
+// With moving walls included (mvs, from tile_summary_kernel), a node with a SOLID pull neighbour
+// whose bc velocity is non-zero or whose bc density is not finite is marked NB_MOVING, and so is a
+// node with a decorated SOLID neighbour (its flag is read again in the step). Next to any other
+// SOLID node the moving-wall term 6 w rho (c . u) is a signed zero, which the step does not add:
+// it can change only the sign of a zero population, and the collision that follows (f - f_eq,
+// with f_eq != 0) does not carry that sign into what it stores. The moving-wall bytes are staged
+// beside the flags in LDS (sfl, then smv). Workgroups loop over tiles (an interior tile costs
+// only its kind).
+template <class L>
+__global__ void __launch_bounds__(1024) node_mask_kernel(const uint8_t* __restrict__ flags,
+                                                         const uint8_t* __restrict__ mvs,
+                                                         const uint32_t* __restrict__ btiles,
+                                                         uint32_t* __restrict__ nbmask, Geo g, uint32_t halo_cells) {
+    extern __shared__ uint8_t sfl[];
+    uint8_t* smv = sfl + halo_cells;
+    const uint32_t n_boundary = btiles[g.n_tiles];
+    uint32_t i = blockIdx.x;
+    if (i >= n_boundary) return;
+    TileCtx<L, uint32_t> t;
+    t.init_pos(g, btiles[i], 0, g.tzb);
+    t.fetch_flags(g, flags, mvs);
+    while (true) {
+        t.commit_flags(g, sfl, smv);
+        uint32_t m = lds_mask(t, sfl);
+        if (mvs && !(m & NB_INVALID)) {
+            bool mv = (m & NB_DECOR) != 0;
+#pragma unroll
+            for (int q = 1; q < L::Q; ++q) mv = mv || t.pull_flag(smv, q) != 0;
+            if (mv) m |= NB_MOVING;
+        }
+        // the next tile's flags are fetched while this tile's mask is stored
+        const uint32_t next = i + gridDim.x;
+        const bool more = next < n_boundary;  // uniform
+        TileCtx<L, uint32_t> tn;
+        if (more) {
+            tn.init_pos(g, btiles[next], 0, g.tzb);
+            tn.fetch_flags(g, flags, mvs);
+        }
+        nbmask[(size_t)t.tile * t.tc + t.loc] = m;
+        if (!more) break;
+        __syncthreads();  // this tile's LDS reads are done before the next tile's flags overwrite them
+        t = tn;
+        i = next;
     }
 }
 
-// The per-tile kind (bit 0: the tile or its one-node halo holds a non-FLUID node, bit 1: a
-// moving SOLID node) and, for boundary tiles, every node's neighbour mask: set_adjacent_flags
-// (src/lbm/kernels/utils/set_flags.mojo) done once per flag state instead of in every step. One
-// workgroup per tile (tile_cells lanes), the tile's flags and their one-node halo staged in LDS.
-// With moving walls included, a node with a SOLID pull neighbour whose bc velocity is non-zero or
-// whose bc density is not finite is marked NB_MOVING, and so is a node with a decorated SOLID
-// neighbour (its flag is read again in the step). Next to any other SOLID node the moving-wall
-// term 6 w rho (c . u) is a signed zero, which the step does not add: it can change only the sign
-// of a zero population, and the collision that follows (f - f_eq, with f_eq != 0) does not carry
-// that sign into what it stores.
-template <class L, class R>
-__global__ void __launch_bounds__(1024) node_mask_kernel(const uint8_t* __restrict__ flags,
-                                                         const R* __restrict__ bc, int moving,
-                                                         const uint32_t* __restrict__ summary,
-                                                         uint32_t* __restrict__ tile_kind,
-                                                         uint32_t* __restrict__ nbmask, Geo g) {
-    constexpr int D = L::D;
-    extern __shared__ uint8_t sfl[];
-    TileCtx<L, uint64_t> t;  // 64-bit bc indices
-    t.init_pos(g, blockIdx.x, 0, g.tzb);
-    const uint32_t kind = boundary_tile<D>(g, summary, t.bx, t.by, t.bz);
-    if (threadIdx.x == 0) tile_kind[t.tile] = kind;
-    if (!(kind & 1u)) return;  // uniform
-    t.fetch_flags(g, flags);
-    t.commit_flags(g, sfl);
-    uint32_t m = lds_mask(t, sfl);
-    if (moving && !(m & NB_INVALID)) {
-        bool mv = (m & NB_DECOR) != 0;
-#pragma unroll
-        for (int q = 1; q < L::Q; ++q) {
-            if ((t.pull_flag(sfl, q) & 0x0F) != LBM_FLAG_SOLID) continue;
-            bool resting = isfinite(bc[t.pull_index(g, q, D + 1, D)]);
-#pragma unroll
-            for (int d = 0; d < D; ++d) resting = resting && bc[t.pull_index(g, q, D + 1, d)] == (R)0;
-            mv = mv || !resting;
-        }
-        if (mv) m |= NB_MOVING;
-    }
-    nbmask[(size_t)t.tile * t.tc + t.loc] = m;
+// Workgroups of node_mask_kernel (whole tiles): one per CU slot at full occupancy, each looping
+// over the boundary tiles. (One-wave z slabs measured slower: their halos hold 2.4 times the cells.)
+constexpr uint32_t kMaskGrid = 1024;
+
+// The kind of every tile from the region masks (boundary_tile), one thread per tile (the 27
+// summary words of a tile are independent loads of one lane), and the list of boundary tiles
+// btiles[0 .. btiles[n_tiles]) for node_mask_kernel (in no particular order; the count starts
+// at zero, set by tile_summary_kernel).
+template <int D>
+__global__ void __launch_bounds__(256) tile_kind_kernel(const uint32_t* __restrict__ summary,
+                                                        uint32_t* __restrict__ tile_kind,
+                                                        uint32_t* __restrict__ btiles, Geo g) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= (uint32_t)g.n_tiles) return;
+    const int bx = (int)(t % (uint32_t)g.Tx), byz = (int)(t / (uint32_t)g.Tx);
+    const uint32_t kind = boundary_tile<D>(g, summary, bx, byz % g.Ty, byz / g.Ty);
+    tile_kind[t] = kind;
+    if (kind) btiles[atomicAdd(&btiles[g.n_tiles], 1u)] = t;
 }
 
 // double_buffer_kernel (src/lbm/kernels/double_buffer.mojo:25-96) and
@@ -914,7 +947,7 @@ __device__ __forceinline__ uint32_t macro_tile(const Geo& g) { return (uint32_t)
 // BC (the BCs take no tau), then moments, or Q = 0.25 |omega|^2 - 0.5 |S|^2 with omega from
 // finite differences of the velocity field and S from the re-streamed populations.
 // Solid nodes (raw flag == SOLID) report bc in the fields and 0 as Q.
-// Interior tiles (own nodes and halo all FLUID, from the tile summary) need no flags: no LDS flag
+// Interior tiles (own nodes and halo all FLUID, from the tile kinds) need no flags: no LDS flag
 // tile, no barrier, no boundary work.
 template <class L, class S, int MODE, bool DDF, int KIND, bool BIG>
 __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
@@ -927,7 +960,7 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
     TileCtx<L, Idx<BIG>> t;
     t.init_pos(p.g, macro_tile(p.g), 0, p.g.tzb);
     const Field<S, BIG> f(p.f, p.f_bytes);
-    const bool boundary = p.summary ? (boundary_tile<L::D>(p.g, p.summary, t.bx, t.by, t.bz) & 1u) != 0u : true;
+    const bool boundary = p.tile_kind ? (p.tile_kind[t.tile] & 1u) != 0u : true;
     if (boundary) t.fetch_flags(p.g, p.flags);
     R fv[Q];
     load_populations<L, S, MODE, BIG>(t, f, fv);

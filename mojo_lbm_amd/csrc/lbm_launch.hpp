@@ -19,6 +19,8 @@ struct Handle {
     uint32_t* summary = nullptr; // per-tile region masks, rebuilt before every step
     uint32_t* tile_kind = nullptr;  // per tile: boundary / moving-wall bits (node_mask_kernel)
     uint32_t* nbmask = nullptr;     // per node: neighbour mask (read for boundary tiles only)
+    uint8_t* mvs = nullptr;         // per node: 1 for a moving SOLID node (moving walls included)
+    uint32_t* btiles = nullptr;     // boundary tile ids, then their count (n_tiles + 1 words)
     bool static_flags = false;   // LBM_OPT_STATIC_FLAGS: rebuild only for new flags / bc buffers
     mutable const void* summary_flags = nullptr;  // buffers the summary was built from
     mutable const void* summary_bc = nullptr;

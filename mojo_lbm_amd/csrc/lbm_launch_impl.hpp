@@ -3,6 +3,8 @@
 // the addressing width. Included only by the per-pair instantiation units.
 #pragma once
 
+#include <algorithm>
+
 #include "lbm_launch.hpp"
 
 namespace lbm {
@@ -76,12 +78,16 @@ template <class L, class S> struct LaunchImpl {
     static int ensure_summary(const Handle& h, const uint8_t* flags, const void* bc, hipStream_t st) {
         const uint32_t n_chunks = (uint32_t)(((size_t)h.g.n_tiles * h.g.tile_cells) >> 4);
         if (!h.static_flags || h.summary_flags != (const void*)flags || h.summary_bc != bc) {
+            uint8_t* mvs = h.d.include_moving_boundary ? h.mvs : nullptr;
             hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, flags,
-                               (const R*)bc, (int)(h.d.include_moving_boundary != 0), h.summary, h.g, n_chunks);
+                               (const R*)bc, mvs, h.summary, h.btiles, h.g, n_chunks);
             if (int rc = check_launch("tile_summary_kernel")) return rc;
-            hipLaunchKernelGGL((node_mask_kernel<L, R>), dim3(h.g.n_tiles), dim3(h.g.tile_cells), h.lds_bytes, st,
-                               flags, (const R*)bc, (int)(h.d.include_moving_boundary != 0), h.summary, h.tile_kind,
-                               h.nbmask, h.g);
+            hipLaunchKernelGGL((tile_kind_kernel<L::D>), dim3((h.g.n_tiles + 255) / 256), dim3(256), 0, st, h.summary,
+                               h.tile_kind, h.btiles, h.g);
+            if (int rc = check_launch("tile_kind_kernel")) return rc;
+            hipLaunchKernelGGL((node_mask_kernel<L>), dim3(std::min<uint32_t>(kMaskGrid, (uint32_t)h.g.n_tiles)),
+                               dim3(h.g.tile_cells), 2 * h.lds_bytes, st, flags, (const uint8_t*)mvs, h.btiles,
+                               h.nbmask, h.g, (uint32_t)h.lds_bytes);
             if (int rc = check_launch("node_mask_kernel")) return rc;
             h.summary_flags = flags;
             h.summary_bc = bc;
@@ -153,7 +159,7 @@ template <class L, class S> struct LaunchImpl {
         // interior tiles skip the flags when the summary can be built (16-byte aligned flags)
         if (!((uintptr_t)flags & 15u)) {
             *rc = ensure_summary(h, flags, bc, st);
-            p.summary = h.summary;
+            p.tile_kind = h.tile_kind;
         }
         p.f = f;
         p.bc = (const R*)bc;

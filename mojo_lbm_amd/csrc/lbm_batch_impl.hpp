@@ -81,16 +81,16 @@ template <class L, class S> struct BatchLaunchImpl {
             const uint32_t n_chunks = (uint32_t)(((size_t)g.n_tiles * g.tile_cells) >> 4);
             uint8_t* mv = b.proto.include_moving_boundary ? b.mvs + b.fl_off[c] : nullptr;
             hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, fl, bcp,
-                               mv, sm, b.btiles, g, n_chunks);
+                               mv, sm, b.btiles, (uint8_t*)nullptr, (uint32_t*)nullptr, g, n_chunks);
             if (int rc = check_launch("tile_summary_kernel")) return rc;
             uint32_t* tk = b.tile_kind + b.tile_begin[c];
             uint32_t* nm = b.nbmask + (size_t)b.tile_begin[c] * b.tile_cells;
             hipLaunchKernelGGL((tile_kind_kernel<L::D>), dim3((g.n_tiles + 255) / 256), dim3(256), 0, st, sm, tk,
-                               b.btiles, g);
+                               b.btiles, (const uint32_t*)nullptr, g);
             if (int rc = check_launch("tile_kind_kernel")) return rc;
             hipLaunchKernelGGL((node_mask_kernel<L>), dim3(std::min<uint32_t>(kMaskGrid, (uint32_t)g.n_tiles)),
-                               dim3(g.tile_cells), 2 * b.lds_bytes, st, fl, (const uint8_t*)mv, b.btiles, nm, g,
-                               (uint32_t)b.lds_bytes);
+                               dim3(g.tile_cells), 2 * b.lds_bytes, st, fl, (const uint8_t*)mv, b.btiles,
+                               (const uint32_t*)nullptr, nm, g, (uint32_t)b.lds_bytes);
             if (int rc = check_launch("node_mask_kernel")) return rc;
             StepParams<R> p{};
             p.f_in = (const T*)b.f[0] + b.f_off[c];

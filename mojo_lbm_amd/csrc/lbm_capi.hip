@@ -204,7 +204,11 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out) {
     if (hipMalloc((void**)&h->tile_kind, (size_t)h->g.n_tiles * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&h->nbmask, (size_t)h->g.n_tiles * h->g.tile_cells * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&h->mvs, (size_t)h->g.n_tiles * h->g.tile_cells) != hipSuccess ||
-        hipMalloc((void**)&h->btiles, ((size_t)h->g.n_tiles + 1) * sizeof(uint32_t)) != hipSuccess) {
+        hipMalloc((void**)&h->btiles, ((size_t)h->g.n_tiles + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&h->flags_seen, (size_t)h->g.n_tiles * h->g.tile_cells) != hipSuccess ||
+        hipMalloc((void**)&h->mask_changed, sizeof(uint32_t)) != hipSuccess) {
+        if (h->flags_seen) (void)hipFree(h->flags_seen);
+        if (h->btiles) (void)hipFree(h->btiles);
         if (h->mvs) (void)hipFree(h->mvs);
         if (h->nbmask) (void)hipFree(h->nbmask);
         if (h->tile_kind) (void)hipFree(h->tile_kind);
@@ -212,6 +216,13 @@ int lbm_create(const lbm_desc* desc, lbm_handle** out) {
         (void)hipFree(h->partial);
         delete h;
         return set_error(LBM_ERR_HIP, "hipMalloc of the neighbour masks failed");
+    }
+    // the first check always builds: changed = 1, no boundary tiles listed yet
+    const uint32_t one = 1u, zero = 0u;
+    if (hipMemcpy(h->mask_changed, &one, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->btiles + h->g.n_tiles, &zero, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)lbm_destroy(reinterpret_cast<lbm_handle*>(h));
+        return set_error(LBM_ERR_HIP, "initialising the mask state failed");
     }
     *out = reinterpret_cast<lbm_handle*>(h);
     return LBM_OK;
@@ -226,6 +237,8 @@ int lbm_destroy(lbm_handle* hh) {
     if (h->nbmask) (void)hipFree(h->nbmask);
     if (h->mvs) (void)hipFree(h->mvs);
     if (h->btiles) (void)hipFree(h->btiles);
+    if (h->flags_seen) (void)hipFree(h->flags_seen);
+    if (h->mask_changed) (void)hipFree(h->mask_changed);
     if (h->link_nodes) (void)hipFree(h->link_nodes);
     if (h->link_q) (void)hipFree(h->link_q);
     if (h->force_partial) (void)hipFree(h->force_partial);

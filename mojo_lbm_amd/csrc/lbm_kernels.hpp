@@ -633,14 +633,19 @@ template <int Q> struct MaxThreads { static constexpr int value = (Q == 9) ? 102
 // Each lane covers 16 consecutive nodes of one tile (tile_cells is a power of two in [32, 1024]);
 // a tile is tile_cells / 16 consecutive lanes of one wave, OR-reduced by shuffles. n_chunks =
 // cells / 16 lanes in total.
+//
+// With seen (the flags the masks were last built from, then their moving-wall bytes in mvs), a
+// chunk whose flags or moving-wall bytes differ from those sets changed[0] and updates seen and
+// mvs; tile_kind_kernel and node_mask_kernel then rebuild, and otherwise return at once. Without
+// seen (a build that always rebuilds) the boundary-tile count is reset here.
 template <int D, class R>
 __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __restrict__ flags,
                                                            const R* __restrict__ bc, uint8_t* __restrict__ mvs,
                                                            uint32_t* __restrict__ summary,
-                                                           uint32_t* __restrict__ btiles, Geo g,
-                                                           uint32_t n_chunks) {
+                                                           uint32_t* __restrict__ btiles, uint8_t* __restrict__ seen,
+                                                           uint32_t* __restrict__ changed, Geo g, uint32_t n_chunks) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;  // 16-byte chunk
-    if (i == 0) btiles[g.n_tiles] = 0;  // boundary-tile count of the tile_kind_kernel that follows
+    if (i == 0 && !seen) btiles[g.n_tiles] = 0;  // boundary-tile count of the tile_kind_kernel that follows
     uint32_t mask = 0;
     if (i < n_chunks) {
         const uint4 v = reinterpret_cast<const uint4*>(flags)[i];
@@ -686,7 +691,18 @@ __global__ void __launch_bounds__(256) tile_summary_kernel(const uint8_t* __rest
                 mv = make_uint4(mw[0], mw[1], mw[2], mw[3]);
             }
         }
-        if (mvs) reinterpret_cast<uint4*>(mvs)[i] = mv;
+        if (seen) {
+            const uint4 sv = reinterpret_cast<const uint4*>(seen)[i];
+            const uint4 om = mvs ? reinterpret_cast<const uint4*>(mvs)[i] : mv;
+            if (((sv.x ^ v.x) | (sv.y ^ v.y) | (sv.z ^ v.z) | (sv.w ^ v.w) | (om.x ^ mv.x) | (om.y ^ mv.y) |
+                 (om.z ^ mv.z) | (om.w ^ mv.w)) != 0u) {
+                reinterpret_cast<uint4*>(seen)[i] = v;
+                if (mvs) reinterpret_cast<uint4*>(mvs)[i] = mv;
+                atomicOr(changed, 1u);
+            }
+        } else if (mvs) {
+            reinterpret_cast<uint4*>(mvs)[i] = mv;
+        }
     }
     const uint32_t lanes_per_tile = (uint32_t)g.tile_cells >> 4;  // 2 .. 64
     for (uint32_t off = 1; off < lanes_per_tile; off <<= 1) mask |= (uint32_t)__shfl_xor((int)mask, (int)off, 64);
@@ -709,9 +725,11 @@ template <class L>
 __global__ void __launch_bounds__(1024) node_mask_kernel(const uint8_t* __restrict__ flags,
                                                          const uint8_t* __restrict__ mvs,
                                                          const uint32_t* __restrict__ btiles,
+                                                         const uint32_t* __restrict__ changed,
                                                          uint32_t* __restrict__ nbmask, Geo g, uint32_t halo_cells) {
     extern __shared__ uint8_t sfl[];
     uint8_t* smv = sfl + halo_cells;
+    if (changed && !*changed) return;  // the masks of these flags are current
     const uint32_t n_boundary = btiles[g.n_tiles];
     uint32_t i = blockIdx.x;
     if (i >= n_boundary) return;
@@ -750,17 +768,28 @@ constexpr uint32_t kMaskGrid = 1024;
 // The kind of every tile from the region masks (boundary_tile), one thread per tile (the 27
 // summary words of a tile are independent loads of one lane), and the list of boundary tiles
 // btiles[0 .. btiles[n_tiles]) for node_mask_kernel (in no particular order; the count starts
-// at zero, set by tile_summary_kernel).
+// at zero, set by tile_summary_kernel or mask_done_kernel).
 template <int D>
 __global__ void __launch_bounds__(256) tile_kind_kernel(const uint32_t* __restrict__ summary,
                                                         uint32_t* __restrict__ tile_kind,
-                                                        uint32_t* __restrict__ btiles, Geo g) {
+                                                        uint32_t* __restrict__ btiles,
+                                                        const uint32_t* __restrict__ changed, Geo g) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= (uint32_t)g.n_tiles) return;
+    if (t >= (uint32_t)g.n_tiles || (changed && !*changed)) return;
     const int bx = (int)(t % (uint32_t)g.Tx), byz = (int)(t / (uint32_t)g.Tx);
     const uint32_t kind = boundary_tile<D>(g, summary, bx, byz % g.Ty, byz / g.Ty);
     tile_kind[t] = kind;
     if (kind) btiles[atomicAdd(&btiles[g.n_tiles], 1u)] = t;
+}
+
+// After node_mask_kernel: clears the change flag and the boundary-tile count for the next check.
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(64) mask_done_kernel(uint32_t* __restrict__ changed, uint32_t* __restrict__ btiles,
+                                                       uint32_t n_tiles) {
+    if (threadIdx.x == 0) {
+        *changed = 0u;
+        btiles[n_tiles] = 0u;
+    }
 }
 
 // double_buffer_kernel (src/lbm/kernels/double_buffer.mojo:25-96) and

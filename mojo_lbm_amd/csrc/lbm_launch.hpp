@@ -21,6 +21,8 @@ struct Handle {
     uint32_t* nbmask = nullptr;     // per node: neighbour mask (read for boundary tiles only)
     uint8_t* mvs = nullptr;         // per node: 1 for a moving SOLID node (moving walls included)
     uint32_t* btiles = nullptr;     // boundary tile ids, then their count (n_tiles + 1 words)
+    uint8_t* flags_seen = nullptr;  // per node: the flags the masks were built from
+    uint32_t* mask_changed = nullptr;  // set when the flags or moving-wall bytes changed (cleared after the rebuild)
     bool static_flags = false;   // LBM_OPT_STATIC_FLAGS: rebuild only for new flags / bc buffers
     mutable const void* summary_flags = nullptr;  // buffers the summary was built from
     mutable const void* summary_bc = nullptr;

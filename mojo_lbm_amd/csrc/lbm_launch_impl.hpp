@@ -73,22 +73,26 @@ template <class L, class S> struct LaunchImpl {
     }
 
     // The per-tile flag summary of (flags, bc), the tile kinds and the neighbour masks of the
-    // boundary tiles: rebuilt unless static flags are promised and they were built from these
-    // buffers.
+    // boundary tiles: checked before a step unless static flags are promised and they were built
+    // from these buffers; rebuilt when the flags or the moving-wall bytes differ from the ones
+    // they were built from (the kind and mask kernels return at once otherwise).
     static int ensure_summary(const Handle& h, const uint8_t* flags, const void* bc, hipStream_t st) {
         const uint32_t n_chunks = (uint32_t)(((size_t)h.g.n_tiles * h.g.tile_cells) >> 4);
         if (!h.static_flags || h.summary_flags != (const void*)flags || h.summary_bc != bc) {
             uint8_t* mvs = h.d.include_moving_boundary ? h.mvs : nullptr;
             hipLaunchKernelGGL((tile_summary_kernel<L::D, R>), dim3((n_chunks + 255) / 256), dim3(256), 0, st, flags,
-                               (const R*)bc, mvs, h.summary, h.btiles, h.g, n_chunks);
+                               (const R*)bc, mvs, h.summary, h.btiles, h.flags_seen, h.mask_changed, h.g, n_chunks);
             if (int rc = check_launch("tile_summary_kernel")) return rc;
             hipLaunchKernelGGL((tile_kind_kernel<L::D>), dim3((h.g.n_tiles + 255) / 256), dim3(256), 0, st, h.summary,
-                               h.tile_kind, h.btiles, h.g);
+                               h.tile_kind, h.btiles, (const uint32_t*)h.mask_changed, h.g);
             if (int rc = check_launch("tile_kind_kernel")) return rc;
             hipLaunchKernelGGL((node_mask_kernel<L>), dim3(std::min<uint32_t>(kMaskGrid, (uint32_t)h.g.n_tiles)),
                                dim3(h.g.tile_cells), 2 * h.lds_bytes, st, flags, (const uint8_t*)mvs, h.btiles,
-                               h.nbmask, h.g, (uint32_t)h.lds_bytes);
+                               (const uint32_t*)h.mask_changed, h.nbmask, h.g, (uint32_t)h.lds_bytes);
             if (int rc = check_launch("node_mask_kernel")) return rc;
+            hipLaunchKernelGGL(mask_done_kernel<0>, dim3(1), dim3(64), 0, st, h.mask_changed, h.btiles,
+                               (uint32_t)h.g.n_tiles);
+            if (int rc = check_launch("mask_done_kernel")) return rc;
             h.summary_flags = flags;
             h.summary_bc = bc;
         }

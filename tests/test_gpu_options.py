@@ -78,6 +78,44 @@ def test_static_flags_follow_in_place_edits(gpu):
     assert np.array_equal(run.f_host(), b)
 
 
+def test_masks_follow_in_place_edits_without_static_flags(gpu):
+    """Without static flags the neighbour masks are rebuilt only when the flags or a wall's moving
+    state change: flags edited in place (an obstacle), then only the bc of a resting wall (it starts
+    moving), then nothing; every step bit-identical to the oracle, esoteric pull and double buffer."""
+    import torch
+
+    for method in (0, 1):
+        desc = U.make_desc("D3Q19", (16, 16, 16), method=method)
+        print(f"case in-place edits, no static flags, method {method}", flush=True)
+        f, bc, flags = U.perturbed_state(desc, seed=21)  # LDC walls, moving lid
+        run = U.DeviceRun(desc, f, bc, flags)
+        a, b = f.copy(), f.copy()
+
+        def step(k, bc_, flags_):
+            nonlocal a, b
+            if method == 0:
+                run.step_db(0.7)
+                O.step_db(desc, b, a, bc_, flags_, 0.7)
+                a, b = b, a
+            else:
+                run.step_ep(0.7, k % 2 == 0)
+                O.step_ep(desc, a, bc_, flags_, 0.7, k % 2 == 0)
+            assert np.array_equal(run.f_host().view(np.uint8), a.view(np.uint8)), f"differs after step {k + 1}"
+
+        step(0, bc, flags)
+        flags2 = flags.copy()
+        amap = U.index_map(desc, 1)[..., 0]
+        flags2[amap[5:9, 4:8, 6:10].ravel()] = 1
+        run.flags.copy_(torch.from_numpy(flags2))  # obstacle, same buffer
+        step(1, bc, flags2)
+        bc2 = bc.copy()
+        side = U.index_map(desc, 4)[0, 1:15, 1:15, 1].ravel()  # the -X wall starts moving along y
+        bc2[side] = 0.02
+        run.bc.copy_(torch.from_numpy(bc2))
+        step(2, bc2, flags2)
+        step(3, bc2, flags2)  # nothing changed
+
+
 ADDR64_CASES = [("D2Q9", 0, 0, 0), ("D2Q9", 1, 1, 1), ("D3Q19", 0, 0, 0), ("D3Q19", 1, 1, 1), ("D3Q19", 2, 0, 1),
                 ("D3Q27", 0, 1, 0), ("D3Q27", 2, 0, 0)]
 

@@ -339,9 +339,15 @@ int lbm_init_f_from_fields(lbm_handle* h, void* f, double rho, const void* u, co
  *                      the learned closure on 3-D tiles of more than 256 nodes,
  *                      else one workgroup per tile. Results are identical for
  *                      every value.
+ *   LBM_OPT_STAGED_PULL: 1 (default) = Float16C double-buffer steps on tiles 8 x 8
+ *                      nodes wide in 256-lane workgroups (32-bit addressing, no
+ *                      interpolated links) copy each direction's source lines into
+ *                      LDS first and pull from there; 0 = pull straight from memory.
+ *                      Results are identical either way.
  * Returns LBM_ERR_INVALID for an unknown option or value.
  */
-enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2, LBM_OPT_ADDR64 = 3, LBM_OPT_STEP_LANES = 4 };
+enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2, LBM_OPT_ADDR64 = 3, LBM_OPT_STEP_LANES = 4,
+       LBM_OPT_STAGED_PULL = 5 };
 int lbm_set_option(lbm_handle* h, int option, int value);
 
 /* Device memory helpers for hosts without their own GPU runtime (the Mojo host

@@ -17,7 +17,7 @@ COLL_SRT, COLL_TRT, COLL_KBC, COLL_RLBM = 0, 1, 2, 3
 STORE_F32, STORE_F16C, STORE_F64 = 0, 1, 2
 LES_NONE, LES_SMAGORINSKY, LES_MLP = 0, 1, 2
 U_XYZD, U_DXYZ = 0, 1
-OPT_XCD_REMAP, OPT_STATIC_FLAGS, OPT_ADDR64, OPT_STEP_LANES = 1, 2, 3, 4
+OPT_XCD_REMAP, OPT_STATIC_FLAGS, OPT_ADDR64, OPT_STEP_LANES, OPT_STAGED_PULL = 1, 2, 3, 4, 5
 BATCH_OPT_GRID_BLOCKS, BATCH_OPT_GRAPH_STEPS = 2, 3
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_NO_DEVICE = 0, -1, -2, -3, -4
 

@@ -509,6 +509,11 @@ int lbm_set_option(lbm_handle* hh, int option, int value) {
         h.g.wzb = wzb;
         return LBM_OK;
     }
+    if (option == LBM_OPT_STAGED_PULL) {
+        if (value != 0 && value != 1) return set_error(LBM_ERR_INVALID, "LBM_OPT_STAGED_PULL takes 0 or 1");
+        h.staged_pull = value != 0;
+        return LBM_OK;
+    }
     if (option == LBM_OPT_ADDR64) {
         if (value != 0 && value != 1) return set_error(LBM_ERR_INVALID, "LBM_OPT_ADDR64 takes 0 or 1");
         h.big = value != 0 || h.f_bytes >= (1ull << 31);

@@ -455,6 +455,89 @@ __device__ __forceinline__ void load_populations(const TileCtx<L, Idx<BIG>>& t, 
     }
 }
 
+// Staged pull (Float16C double buffer, 3-D tiles 8 x 8 nodes wide, 256-lane workgroups of four
+// z layers). A 64-lane wave that pulls its 8 x 8 layer straight from memory touches the tile's
+// own 128-byte line and, for a direction with an x or y component, the line of the neighbour
+// tile as well, 2 bytes per lane: on the 256^3 cube the population loads then run at about
+// 80 % of the rate of aligned line loads. Here the workgroup first copies each q plane's source
+// rows into LDS, 16 bytes (one 8-node row) per lane, eight lanes per 8-row block, straight from
+// memory to LDS, and each lane then reads its pull from LDS.
+//
+// Plane q's box holds, for every layer l of the slab and y row r of the tile, the row
+// y - c_qy of layer z - c_qz (the lane's own row and layer moved by -c_q, in the neighbour tile
+// where that leaves the tile): one 16-byte row for c_qx = 0, and for c_qx != 0 also the row of
+// the x-neighbour tile column the pull reaches into (tile columns bx - 1, bx for c_qx = 1; bx,
+// bx + 1 for c_qx = -1). Rows in (layer, tile column, row) order, planes one after the other.
+template <class L> struct StagedBox {
+    static constexpr int SZ = 4;  // slab layers
+    static constexpr int ntx(int q) { return 1 + (L::c(q, 0) != 0); }
+    static constexpr int rows(int q) { return 8 * SZ * ntx(q); }  // 16-byte rows
+    static constexpr int offset(int q) {  // bytes
+        int s = 0;
+        for (int p = 0; p < q; ++p) s += rows(p) * 16;
+        return s;
+    }
+    static constexpr int bytes() { return offset(L::Q); }
+};
+
+// The loads of plane q's box, 64 rows per wave-sized chunk: chunk i of plane q is taken by wave
+// (first chunk of q + i) mod 4; lanes past the box's rows are idle. The direct-to-LDS load writes
+// lane j's 16 bytes at (chunk address + 16 j), so row order in LDS is lane order.
+template <class L, class TC, class FV>
+__device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void* f_in, FV& fv) {
+    using B = StagedBox<L>;
+    constexpr int Q = L::Q;
+    extern __shared__ __attribute__((aligned(16))) uint8_t staged_box[];
+    const int wave = __builtin_amdgcn_readfirstlane(t.tid >> 6), lane = t.tid & 63;
+    const uint16_t* f = (const uint16_t*)f_in;
+    const uint32_t TQ = t.tc * (uint32_t)Q;
+    const int tzm = (1 << g.tzb) - 1;
+    // tile rows below / above (periodic)
+    const int by_m = t.by > 0 ? t.by - 1 : g.Ty - 1, by_p = t.by < g.Ty - 1 ? t.by + 1 : 0;
+    int chunk = 0;  // chunks of the planes before q
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
+        // first tile column of the box (periodic) and the global z of the slab's first source layer
+        int bx0 = t.bx - (cx == 1 ? 1 : 0);
+        bx0 += bx0 < 0 ? g.Tx : 0;
+        const int z0 = (t.bz << g.tzb) + t.wz0 - cz;
+        const int nch = (B::rows(q) + 63) >> 6;
+#pragma unroll
+        for (int i = 0; i < nch; ++i, ++chunk) {
+            if ((chunk & 3) != wave) continue;  // wave-uniform
+            const int j = i * 64 + lane;
+            if (j < B::rows(q)) {
+                const int r = j & 7, blk = j >> 3;
+                const int tx = blk % B::ntx(q), l = blk / B::ntx(q);
+                const int sy = r - cy;  // source row, -1 .. 8
+                const int yt = sy < 0 ? by_m : (sy > 7 ? by_p : t.by);
+                int xt = bx0 + tx;
+                xt -= xt >= g.Tx ? g.Tx : 0;
+                int zl = z0 + l;
+                zl += zl < 0 ? g.nz : 0;
+                zl -= zl >= g.nz ? g.nz : 0;
+                const uint32_t tile = ((uint32_t)(zl >> g.tzb) * (uint32_t)g.Ty + (uint32_t)yt) * (uint32_t)g.Tx + (uint32_t)xt;
+                const uint32_t e = tile * TQ + (uint32_t)q * t.tc + ((uint32_t)(zl & tzm) << 6) + ((uint32_t)(sy & 7) << 3);
+#if defined(__HIP_DEVICE_COMPILE__)  // a gfx950 builtin: absent from the host pass
+                __builtin_amdgcn_global_load_lds(f + e, staged_box + B::offset(q) + i * 1024, 16, 0, 0);
+#endif
+            }
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int lx = t.tid & 7, ly = (t.tid >> 3) & 7, lz = t.tid >> 6;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int cx = L::c(q, 0);
+        const int xb = lx - cx + (cx == 1 ? 8 : 0);
+        const int row = (lz * B::ntx(q) + (xb >> 3)) * 8 + ly;
+        const int16_t code = *(const int16_t*)(staged_box + B::offset(q) + ((row * 8 + (xb & 7)) << 1));
+        fv[q] = f16c_decode_sext(code);
+    }
+}
+
 // Store: double buffer writes every population at x; esoteric_pull_store_f_vec
 // (ops/load_and_store.mojo:89-159) writes the parity's slot pattern.
 template <class L, class S, int MODE, bool BIG>
@@ -838,7 +921,7 @@ __global__ void __launch_bounds__(64) mask_done_kernel(uint32_t* __restrict__ ch
 // step_tile updates one tile; force_slot numbers its per-wave force records. LINKS
 // compiles the interpolated bounce-back path in (double buffer only): kept out of the
 // plain kernels, whose register budget it would otherwise set.
-template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS>
+template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS, bool STAGED = false>
 __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, uint32_t tile, uint32_t force_slot,
                                           uint32_t part, int lzb) {
     using R = typename S::R;
@@ -852,7 +935,10 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
     uint32_t m = 0;  // issued before the population loads: its latency overlaps theirs
     if (boundary) m = p.nbmask[(size_t)t.tile * t.tc + t.loc];
     R fv[Q];
-    load_populations<L, S, MODE, BIG>(t, fin, fv);
+    if constexpr (STAGED && S::id == LBM_STORE_F16C && MODE == MODE_DB && !BIG && L::D == 3)
+        stage_pull<L>(t, p.g, p.f_in, fv);
+    else
+        load_populations<L, S, MODE, BIG>(t, fin, fv);
     constexpr bool links = LINKS && MODE == MODE_DB;
     double force[3] = {0.0, 0.0, 0.0};
     bool active = true;
@@ -898,6 +984,16 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParam
     const int s = p.g.tzb - p.g.wzb;
     const uint32_t unit = (uint32_t)tile_of_block(p.g, s);
     step_tile<L, S, MODE, COLL, LES, DDF, BIG, LINKS>(p, unit >> s, unit, unit & ((1u << s) - 1u), p.g.wzb);
+}
+
+// The staged-pull step (Float16C, double buffer, 8 x 8 x tz tiles in 256-lane z slabs of four
+// layers, 32-bit addressing, no interpolated links); dynamic LDS: StagedBox<L>::bytes().
+template <class L, int COLL, int LES, bool DDF>
+__global__ void __launch_bounds__(256) step_kernel_staged(StepParams<float> p) {
+    const int s = p.g.tzb - p.g.wzb;
+    const uint32_t unit = (uint32_t)tile_of_block(p.g, s);
+    step_tile<L, StoreF16C, MODE_DB, COLL, LES, DDF, false, false, true>(p, unit >> s, unit, unit & ((1u << s) - 1u),
+                                                                         p.g.wzb);
 }
 
 // The step with the learned closure, held to at most 128 VGPRs (two 512-lane workgroups per CU)

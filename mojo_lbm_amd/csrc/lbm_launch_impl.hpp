@@ -17,6 +17,12 @@ template <class L, class S> struct LaunchImpl {
     static int step_l(const Handle& h, const StepParams<R>& p, hipStream_t st) {
         const int s = h.g.tzb - h.g.wzb;  // 2^s workgroups per tile, one z slab each
         const dim3 grid((unsigned)h.g.n_tiles << s), block((unsigned)h.g.tile_cells >> s);
+        if constexpr (S::id == LBM_STORE_F16C && L::D == 3 && MODE == MODE_DB && !LINKS) {
+            if (h.staged_pull && !h.big && h.g.txb == 3 && h.g.tyb == 3 && h.g.wzb == 2) {
+                hipLaunchKernelGGL((step_kernel_staged<L, COLL, LES, DDF>), grid, block, StagedBox<L>::bytes(), st, p);
+                return check_launch("step_kernel_staged");
+            }
+        }
         if (h.big)
             hipLaunchKernelGGL((step_kernel<L, S, MODE, COLL, LES, DDF, true, LINKS>), grid, block, 0, st, p);
         else

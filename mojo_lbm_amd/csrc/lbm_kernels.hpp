@@ -648,7 +648,7 @@ __device__ __forceinline__ uint32_t bouzidi_links(const TileCtx<L, Idx<BIG>>& t,
 
 // GROUPED: the moving-wall bc loads of six directions are issued together (one wait per group
 // instead of one per direction); measured 1.7 % faster on the 256^3 FP32 cube, but the arrays it
-// keeps live raise the step's VGPRs (38 -> 96), which costs the instruction-bound Float16C step 6 %
+// keeps live raise the step's VGPRs (38 -> 96), which costs the Float16C step 6 % (fewer waves)
 // and would spill the learned-closure step, so only the FP32 steps without it group the loads.
 template <class L, class S, int MODE, bool DDF, bool BIG, bool GROUPED = false>
 __device__ __forceinline__ void apply_boundaries(const TileCtx<L, Idx<BIG>>& t, const Geo& g, uint32_t m,

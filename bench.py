@@ -422,6 +422,8 @@ def main():
                     help="run one variant: fp32 | f16c | f16c_ep | fp32_ep | fp32_periodic | f16c_periodic | cylinders | les "
                          "| outputs")
     ap.add_argument("--xcd-remap", type=int, default=None, help="override LBM_OPT_XCD_REMAP (0 or 1)")
+    ap.add_argument("--staged-pull", type=int, default=None,
+                    help="override LBM_OPT_STAGED_PULL (0 direct, 1 per tile, 2 per x-adjacent tile pair)")
     ap.add_argument("--step-lanes", type=int, default=None,
                     help="override LBM_OPT_STEP_LANES (lanes per step workgroup; 0 = one workgroup per tile)")
     ap.add_argument("--calibrate", action="store_true",
@@ -511,6 +513,8 @@ def main():
             _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_XCD_REMAP, args.xcd_remap))
         if args.step_lanes is not None:
             _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_STEP_LANES, args.step_lanes))
+        if args.staged_pull is not None:
+            _lib.check(run["L"].lbm_set_option(run["h"].ptr, _lib.OPT_STAGED_PULL, args.staged_pull))
         steps = args.steps if (name == "fp32" or args.only) else max(100, args.steps // 2)
         t = time_run(run, tau_v, steps, args.warmup)
         B = algorithmic_bytes(19, N, N, N, s, walls)

@@ -341,9 +341,11 @@ int lbm_init_f_from_fields(lbm_handle* h, void* f, double rho, const void* u, co
  *                      every value.
  *   LBM_OPT_STAGED_PULL: 1 (default) = Float16C double-buffer steps on tiles 8 x 8
  *                      nodes wide in 256-lane workgroups (32-bit addressing, no
- *                      interpolated links) copy each direction's source lines into
- *                      LDS first and pull from there; 0 = pull straight from memory.
- *                      Results are identical either way.
+ *                      interpolated links) copy each direction's source rows into
+ *                      LDS first and pull from there; 2 = the same with one 512-lane
+ *                      workgroup per pair of x-adjacent tiles (an even tile count
+ *                      in x; else as 1); 0 = pull straight from memory. Results are
+ *                      identical for every value.
  * Returns LBM_ERR_INVALID for an unknown option or value.
  */
 enum { LBM_OPT_XCD_REMAP = 1, LBM_OPT_STATIC_FLAGS = 2, LBM_OPT_ADDR64 = 3, LBM_OPT_STEP_LANES = 4,

@@ -510,8 +510,8 @@ int lbm_set_option(lbm_handle* hh, int option, int value) {
         return LBM_OK;
     }
     if (option == LBM_OPT_STAGED_PULL) {
-        if (value != 0 && value != 1) return set_error(LBM_ERR_INVALID, "LBM_OPT_STAGED_PULL takes 0 or 1");
-        h.staged_pull = value != 0;
+        if (value < 0 || value > 2) return set_error(LBM_ERR_INVALID, "LBM_OPT_STAGED_PULL takes 0, 1 or 2");
+        h.staged_pull = value;
         return LBM_OK;
     }
     if (option == LBM_OPT_ADDR64) {

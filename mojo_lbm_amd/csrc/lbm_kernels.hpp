@@ -223,10 +223,12 @@ template <class L, class I = uint32_t> struct TileCtx {
 
     // The workgroup covers z layers [part 2^lzb, (part + 1) 2^lzb) of the tile (part 0 and
     // lzb = tzb: the whole tile); lane tid is node tid + part 2^lzb tx ty of that tile.
-    __device__ __forceinline__ void init_pos(const Geo& g, uint32_t tile_id, uint32_t part, int lzb) {
+    // lane_tid: the lane's index in the tile's slab (default: threadIdx.x; a workgroup over several
+    // tiles passes its own).
+    __device__ __forceinline__ void init_pos(const Geo& g, uint32_t tile_id, uint32_t part, int lzb, int lane_tid = -1) {
         constexpr int D = L::D, Q = L::Q;
         tile = (uint32_t)__builtin_amdgcn_readfirstlane(tile_id);
-        tid = threadIdx.x;
+        tid = lane_tid < 0 ? (int)threadIdx.x : lane_tid;
         const int tx = 1 << g.txb, ty = 1 << g.tyb, tz = 1 << g.tzb;
         wz0 = (int)part << lzb;
         wtz = 1 << lzb;
@@ -468,9 +470,9 @@ __device__ __forceinline__ void load_populations(const TileCtx<L, Idx<BIG>>& t, 
 // where that leaves the tile): one 16-byte row for c_qx = 0, and for c_qx != 0 also the row of
 // the x-neighbour tile column the pull reaches into (tile columns bx - 1, bx for c_qx = 1; bx,
 // bx + 1 for c_qx = -1). Rows in (layer, tile column, row) order, planes one after the other.
-template <class L> struct StagedBox {
+template <class L, int BTX = 1> struct StagedBox {
     static constexpr int SZ = 4;  // slab layers
-    static constexpr int ntx(int q) { return 1 + (L::c(q, 0) != 0); }
+    static constexpr int ntx(int q) { return BTX + (L::c(q, 0) != 0); }
     static constexpr int rows(int q) { return 8 * SZ * ntx(q); }  // 16-byte rows
     static constexpr int offset(int q) {  // bytes
         int s = 0;
@@ -481,14 +483,18 @@ template <class L> struct StagedBox {
 };
 
 // The loads of plane q's box, 64 rows per wave-sized chunk: chunk i of plane q is taken by wave
-// (first chunk of q + i) mod 4; lanes past the box's rows are idle. The direct-to-LDS load writes
-// lane j's 16 bytes at (chunk address + 16 j), so row order in LDS is lane order.
-template <class L, class TC, class FV>
+// (first chunk of q + i) mod (waves of the workgroup); lanes past the box's rows are idle. The
+// direct-to-LDS load writes lane j's 16 bytes at (chunk address + 16 j), so row order in LDS is lane
+// order. BTX = 2: the workgroup covers two x-adjacent tiles (waves 0-3 the first, 4-7 the second),
+// whose boxes share their rows, so the first tile's rows are the second's x halo and back.
+template <class L, int BTX, class TC, class FV>
 __device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void* f_in, FV& fv) {
-    using B = StagedBox<L>;
-    constexpr int Q = L::Q;
+    using B = StagedBox<L, BTX>;
+    constexpr int Q = L::Q, NW = 4 * BTX;
     extern __shared__ __attribute__((aligned(16))) uint8_t staged_box[];
-    const int wave = __builtin_amdgcn_readfirstlane(t.tid >> 6), lane = t.tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    const int k = BTX == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 8);  // tile of the wave
+    const int bxa = t.bx - k;  // first tile column of the workgroup
     const uint16_t* f = (const uint16_t*)f_in;
     const uint32_t TQ = t.tc * (uint32_t)Q;
     const int tzm = (1 << g.tzb) - 1;
@@ -499,13 +505,13 @@ __device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void
     for (int q = 0; q < Q; ++q) {
         const int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
         // first tile column of the box (periodic) and the global z of the slab's first source layer
-        int bx0 = t.bx - (cx == 1 ? 1 : 0);
+        int bx0 = bxa - (cx == 1 ? 1 : 0);
         bx0 += bx0 < 0 ? g.Tx : 0;
         const int z0 = (t.bz << g.tzb) + t.wz0 - cz;
         const int nch = (B::rows(q) + 63) >> 6;
 #pragma unroll
         for (int i = 0; i < nch; ++i, ++chunk) {
-            if ((chunk & 3) != wave) continue;  // wave-uniform
+            if (chunk % NW != wave) continue;  // wave-uniform
             const int j = i * 64 + lane;
             if (j < B::rows(q)) {
                 const int r = j & 7, blk = j >> 3;
@@ -531,7 +537,7 @@ __device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int cx = L::c(q, 0);
-        const int xb = lx - cx + (cx == 1 ? 8 : 0);
+        const int xb = 8 * k + lx - cx + (cx == 1 ? 8 : 0);
         const int row = (lz * B::ntx(q) + (xb >> 3)) * 8 + ly;
         const int16_t code = *(const int16_t*)(staged_box + B::offset(q) + ((row * 8 + (xb & 7)) << 1));
         fv[q] = f16c_decode_sext(code);
@@ -921,13 +927,13 @@ __global__ void __launch_bounds__(64) mask_done_kernel(uint32_t* __restrict__ ch
 // step_tile updates one tile; force_slot numbers its per-wave force records. LINKS
 // compiles the interpolated bounce-back path in (double buffer only): kept out of the
 // plain kernels, whose register budget it would otherwise set.
-template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS, bool STAGED = false>
+template <class L, class S, int MODE, int COLL, int LES, bool DDF, bool BIG, bool LINKS, int STAGE = 0>
 __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, uint32_t tile, uint32_t force_slot,
                                           uint32_t part, int lzb) {
     using R = typename S::R;
     constexpr int Q = L::Q;
     TileCtx<L, Idx<BIG>> t;
-    t.init_pos(p.g, tile, part, lzb);
+    t.init_pos(p.g, tile, part, lzb, STAGE == 2 ? (int)(threadIdx.x & 255u) : -1);
     const uint32_t kind = p.tile_kind[t.tile];  // uniform
     const bool boundary = (kind & 1u) != 0;
     const Field<S, BIG> fin(p.f_in, p.f_bytes);
@@ -935,8 +941,8 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
     uint32_t m = 0;  // issued before the population loads: its latency overlaps theirs
     if (boundary) m = p.nbmask[(size_t)t.tile * t.tc + t.loc];
     R fv[Q];
-    if constexpr (STAGED && S::id == LBM_STORE_F16C && MODE == MODE_DB && !BIG && L::D == 3)
-        stage_pull<L>(t, p.g, p.f_in, fv);
+    if constexpr (STAGE > 0 && S::id == LBM_STORE_F16C && MODE == MODE_DB && !BIG && L::D == 3)
+        stage_pull<L, STAGE>(t, p.g, p.f_in, fv);
     else
         load_populations<L, S, MODE, BIG>(t, fin, fv);
     constexpr bool links = LINKS && MODE == MODE_DB;
@@ -987,13 +993,23 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value) step_kernel(StepParam
 }
 
 // The staged-pull step (Float16C, double buffer, 8 x 8 x tz tiles in 256-lane z slabs of four
-// layers, 32-bit addressing, no interpolated links); dynamic LDS: StagedBox<L>::bytes().
-template <class L, int COLL, int LES, bool DDF>
-__global__ void __launch_bounds__(256) step_kernel_staged(StepParams<float> p) {
+// layers, 32-bit addressing, no interpolated links); dynamic LDS: StagedBox<L, BTX>::bytes().
+// BTX = 2: one workgroup per slab of two x-adjacent tiles (Tx even), numbered pair-major, with the
+// XCD remap over pairs when the tiles per XCD are even (else hardware order).
+template <class L, int COLL, int LES, bool DDF, int BTX>
+__global__ void __launch_bounds__(256 * BTX) step_kernel_staged(StepParams<float> p) {
     const int s = p.g.tzb - p.g.wzb;
-    const uint32_t unit = (uint32_t)tile_of_block(p.g, s);
-    step_tile<L, StoreF16C, MODE_DB, COLL, LES, DDF, false, false, true>(p, unit >> s, unit, unit & ((1u << s) - 1u),
-                                                                         p.g.wzb);
+    uint32_t unit;
+    if constexpr (BTX == 1) {
+        unit = (uint32_t)tile_of_block(p.g, s);
+    } else {
+        const uint32_t b = blockIdx.x;
+        const uint32_t chunk = (p.g.xcd_chunk & 1) ? 0u : (uint32_t)(p.g.xcd_chunk >> 1) << s;
+        unit = chunk ? (b & 7u) * chunk + (b >> 3) : b;
+    }
+    const uint32_t tile = (unit >> s) * BTX + (BTX == 1 ? 0u : (threadIdx.x >> 8));
+    step_tile<L, StoreF16C, MODE_DB, COLL, LES, DDF, false, false, BTX>(p, tile, unit, unit & ((1u << s) - 1u),
+                                                                        p.g.wzb);
 }
 
 // The step with the learned closure, held to at most 128 VGPRs (two 512-lane workgroups per CU)

@@ -24,7 +24,7 @@ struct Handle {
     uint8_t* flags_seen = nullptr;  // per node: the flags the masks were built from
     uint32_t* mask_changed = nullptr;  // set when the flags or moving-wall bytes changed (cleared after the rebuild)
     bool static_flags = false;   // LBM_OPT_STATIC_FLAGS: rebuild only for new flags / bc buffers
-    bool staged_pull = true;     // LBM_OPT_STAGED_PULL: Float16C double-buffer pulls through LDS
+    int staged_pull = 1;         // LBM_OPT_STAGED_PULL: Float16C double-buffer pulls through LDS (2: tile pairs)
     mutable const void* summary_flags = nullptr;  // buffers the summary was built from
     mutable const void* summary_bc = nullptr;
     uint32_t* link_nodes = nullptr;  // interpolated bounce-back records (handle-owned copies)

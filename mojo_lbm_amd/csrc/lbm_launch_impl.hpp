@@ -19,7 +19,14 @@ template <class L, class S> struct LaunchImpl {
         const dim3 grid((unsigned)h.g.n_tiles << s), block((unsigned)h.g.tile_cells >> s);
         if constexpr (S::id == LBM_STORE_F16C && L::D == 3 && MODE == MODE_DB && !LINKS) {
             if (h.staged_pull && !h.big && h.g.txb == 3 && h.g.tyb == 3 && h.g.wzb == 2) {
-                hipLaunchKernelGGL((step_kernel_staged<L, COLL, LES, DDF>), grid, block, StagedBox<L>::bytes(), st, p);
+                if (h.staged_pull == 2 && h.g.Tx % 2 == 0) {
+                    const dim3 grid2((unsigned)(h.g.n_tiles / 2) << s), block2(512);
+                    const size_t lds2 = StagedBox<L, 2>::bytes();
+                    hipLaunchKernelGGL((step_kernel_staged<L, COLL, LES, DDF, 2>), grid2, block2, lds2, st, p);
+                } else {
+                    const size_t lds1 = StagedBox<L, 1>::bytes();
+                    hipLaunchKernelGGL((step_kernel_staged<L, COLL, LES, DDF, 1>), grid, block, lds1, st, p);
+                }
                 return check_launch("step_kernel_staged");
             }
         }

@@ -218,24 +218,26 @@ STAGED_CASES = [("D3Q19", 0, 0, 1, True, (32, 16, 24), False), ("D3Q19", 0, 1, 0
 
 @pytest.mark.parametrize("lat,coll,les,ddf,walls,grid,eq", STAGED_CASES)
 def test_staged_pull_matches_oracle(gpu, lat, coll, les, ddf, walls, grid, eq):
-    """The staged pull (default) and the direct pull (LBM_OPT_STAGED_PULL 0) against the oracle:
-    populations bit-identical after every step, periodic wrap in x, y and z included (periodic
-    boxes), walls with a moving lid, equilibrium nodes on one face."""
+    """The direct pull (LBM_OPT_STAGED_PULL 0), the staged pull per tile (1) and per x-adjacent tile
+    pair (2) against the oracle: populations bit-identical after every step, periodic wrap in x, y
+    and z included (periodic boxes), walls with a moving lid, equilibrium nodes on one face; an odd
+    tile count in x falls back to one tile per workgroup."""
     print(f"case staged pull {lat} coll={coll} les={les} ddf={ddf} walls={walls} grid={grid} eq={eq}", flush=True)
     desc = U.make_desc(lat, grid, storage=1, ddf=ddf, collision=coll, les=les, eq=eq)
     f, bc, flags = U.perturbed_state(desc, seed=31 + coll, walls=walls)
     if eq:  # the +X face an equilibrium outlet (rho given, u free)
         O.set_exterior_wall(desc, flags, bc, "+X", 2, None, 1.0)
-    staged = U.DeviceRun(desc, f, bc, flags)
-    direct = U.DeviceRun(desc, f, bc, flags)
-    _lib.check(direct.L.lbm_set_option(direct.h.ptr, _lib.OPT_STAGED_PULL, 0))
+    runs = {}
+    for mode in (0, 1, 2):  # direct, one tile per workgroup, x-adjacent tile pairs (even Tx)
+        runs[mode] = U.DeviceRun(desc, f, bc, flags)
+        _lib.check(runs[mode].L.lbm_set_option(runs[mode].h.ptr, _lib.OPT_STAGED_PULL, mode))
     a, b = f.copy(), f.copy()
     for k in range(4):
-        staged.step_db(0.56)
-        direct.step_db(0.56)
+        for r in runs.values():
+            r.step_db(0.56)
         O.step_db(desc, b, a, bc, flags, 0.56)
         a, b = b, a
-        assert np.array_equal(staged.f_host().view(np.uint8), a.view(np.uint8)), f"staged, step {k + 1}"
-        assert np.array_equal(direct.f_host().view(np.uint8), a.view(np.uint8)), f"direct, step {k + 1}"
+        for mode, r in runs.items():
+            assert np.array_equal(r.f_host().view(np.uint8), a.view(np.uint8)), f"staged {mode}, step {k + 1}"
     with pytest.raises(_lib.LbmError):
-        _lib.check(staged.L.lbm_set_option(staged.h.ptr, _lib.OPT_STAGED_PULL, 2))
+        _lib.check(runs[0].L.lbm_set_option(runs[0].h.ptr, _lib.OPT_STAGED_PULL, 3))

@@ -462,18 +462,22 @@ __device__ __forceinline__ void load_populations(const TileCtx<L, Idx<BIG>>& t, 
 // own 128-byte line and, for a direction with an x or y component, the line of the neighbour
 // tile as well, 2 bytes per lane: on the 256^3 cube the population loads then run at about
 // 80 % of the rate of aligned line loads. Here the workgroup first copies each q plane's source
-// rows into LDS, 16 bytes (one 8-node row) per lane, eight lanes per 8-row block, straight from
-// memory to LDS, and each lane then reads its pull from LDS.
+// rows into LDS, 16 bytes (one 8-node row) per lane, straight from memory to LDS, and each lane
+// then reads its pull from LDS.
 //
-// Plane q's box holds, for every layer l of the slab and y row r of the tile, the row
-// y - c_qy of layer z - c_qz (the lane's own row and layer moved by -c_q, in the neighbour tile
-// where that leaves the tile): one 16-byte row for c_qx = 0, and for c_qx != 0 also the row of
-// the x-neighbour tile column the pull reaches into (tile columns bx - 1, bx for c_qx = 1; bx,
-// bx + 1 for c_qx = -1). Rows in (layer, tile column, row) order, planes one after the other.
+// Plane q's box: for every layer and y row of the slab moved by -c_q (the pull sources, from the
+// neighbour tile where that leaves the tile), the 16-byte row of each tile column the pulls reach
+// (one for c_qx = 0; for c_qx = 1 also the column below, for -1 the one above). For c_qz != 0 the
+// box has one more layer, the slab's own layer the move leaves out, staged only in boundary tiles:
+// there the halfway bounce-back reads the node's own population of plane q from the box whenever
+// its row is in it (all but the edge row for c_qy != 0) instead of from memory. Rows in (layer,
+// tile column, row) order, planes one after the other.
 template <class L, int BTX = 1> struct StagedBox {
     static constexpr int SZ = 4;  // slab layers
     static constexpr int ntx(int q) { return BTX + (L::c(q, 0) != 0); }
-    static constexpr int rows(int q) { return 8 * SZ * ntx(q); }  // 16-byte rows
+    static constexpr int nl(int q) { return SZ + (L::c(q, 2) != 0); }           // layers
+    static constexpr int llo(int q) { return L::c(q, 2) == 1 ? -1 : 0; }        // first source layer
+    static constexpr int rows(int q) { return nl(q) * ntx(q) * 8; }             // 16-byte rows
     static constexpr int offset(int q) {  // bytes
         int s = 0;
         for (int p = 0; p < q; ++p) s += rows(p) * 16;
@@ -482,16 +486,17 @@ template <class L, int BTX = 1> struct StagedBox {
     static constexpr int bytes() { return offset(L::Q); }
 };
 
+extern __shared__ __attribute__((aligned(16))) uint8_t staged_box[];
+
 // The loads of plane q's box, 64 rows per wave-sized chunk: chunk i of plane q is taken by wave
-// (first chunk of q + i) mod (waves of the workgroup); lanes past the box's rows are idle. The
-// direct-to-LDS load writes lane j's 16 bytes at (chunk address + 16 j), so row order in LDS is lane
-// order. BTX = 2: the workgroup covers two x-adjacent tiles (waves 0-3 the first, 4-7 the second),
-// whose boxes share their rows, so the first tile's rows are the second's x halo and back.
+// (first chunk of q + i) mod (waves of the workgroup); lanes past the box's rows, and in interior
+// tiles lanes of rows only the bounce-back reads, are idle. The direct-to-LDS load writes lane j's
+// 16 bytes at (chunk address + 16 j), so row order in LDS is lane order. BTX = 2: the workgroup
+// covers two x-adjacent tiles (waves 0-3 the first, 4-7 the second), whose boxes share their rows.
 template <class L, int BTX, class TC, class FV>
-__device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void* f_in, FV& fv) {
+__device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void* f_in, bool boundary, FV& fv) {
     using B = StagedBox<L, BTX>;
-    constexpr int Q = L::Q, NW = 4 * BTX;
-    extern __shared__ __attribute__((aligned(16))) uint8_t staged_box[];
+    constexpr int Q = L::Q, NW = 4 * BTX, SZ = B::SZ;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
     const int k = BTX == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 8);  // tile of the wave
     const int bxa = t.bx - k;  // first tile column of the workgroup
@@ -504,23 +509,25 @@ __device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
-        // first tile column of the box (periodic) and the global z of the slab's first source layer
+        // first tile column of the box (periodic) and the global z of the slab's first layer
         int bx0 = bxa - (cx == 1 ? 1 : 0);
         bx0 += bx0 < 0 ? g.Tx : 0;
-        const int z0 = (t.bz << g.tzb) + t.wz0 - cz;
+        const int zs = (t.bz << g.tzb) + t.wz0;
         const int nch = (B::rows(q) + 63) >> 6;
 #pragma unroll
         for (int i = 0; i < nch; ++i, ++chunk) {
             if (chunk % NW != wave) continue;  // wave-uniform
             const int j = i * 64 + lane;
-            if (j < B::rows(q)) {
-                const int r = j & 7, blk = j >> 3;
-                const int tx = blk % B::ntx(q), l = blk / B::ntx(q);
-                const int sy = r - cy;  // source row, -1 .. 8
+            const int r = j & 7, blk = j >> 3;
+            const int tx = blk % B::ntx(q), l = blk / B::ntx(q);
+            const int sy = r - cy;         // source row, -1 .. 8
+            const int sl = l + B::llo(q);  // source layer relative to the slab, -1 .. SZ
+            const bool pulled = sl >= -cz && sl <= SZ - 1 - cz;
+            if (j < B::rows(q) && (boundary || pulled)) {
                 const int yt = sy < 0 ? by_m : (sy > 7 ? by_p : t.by);
                 int xt = bx0 + tx;
                 xt -= xt >= g.Tx ? g.Tx : 0;
-                int zl = z0 + l;
+                int zl = zs + sl;
                 zl += zl < 0 ? g.nz : 0;
                 zl -= zl >= g.nz ? g.nz : 0;
                 const uint32_t tile = ((uint32_t)(zl >> g.tzb) * (uint32_t)g.Ty + (uint32_t)yt) * (uint32_t)g.Tx + (uint32_t)xt;
@@ -536,11 +543,34 @@ __device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void
     const int lx = t.tid & 7, ly = (t.tid >> 3) & 7, lz = t.tid >> 6;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int cx = L::c(q, 0);
+        const int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
         const int xb = 8 * k + lx - cx + (cx == 1 ? 8 : 0);
-        const int row = (lz * B::ntx(q) + (xb >> 3)) * 8 + ly;
+        const int row = ((lz - cz - B::llo(q)) * B::ntx(q) + (xb >> 3)) * 8 + ly;
         const int16_t code = *(const int16_t*)(staged_box + B::offset(q) + ((row * 8 + (xb & 7)) << 1));
         fv[q] = f16c_decode_sext(code);
+    }
+}
+
+// Halfway bounce-back of a staged boundary tile: the node's own population of plane opp(q) comes
+// from the box of that plane when the node's row is in it (its layer always is), else from memory.
+template <class L, int BTX, class TC, class F, class FV>
+__device__ __forceinline__ void staged_bounce_back(const TC& t, uint32_t m, const F& f, FV& fv) {
+    using B = StagedBox<L, BTX>;
+    const int k = BTX == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 8);
+    const int lx = t.tid & 7, ly = (t.tid >> 3) & 7, lz = t.tid >> 6;
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) {
+        if ((m >> q) & 1u) {
+            const int p = opp(q);
+            const int ro = ly + L::c(p, 1);  // box row of the node's own row
+            if (ro >= 0 && ro < 8) {
+                const int col = k + (L::c(p, 0) == 1 ? 1 : 0);
+                const int row = ((lz - B::llo(p)) * B::ntx(p) + col) * 8 + ro;
+                fv[q] = f16c_decode_sext(*(const int16_t*)(staged_box + B::offset(p) + ((row * 8 + lx) << 1)));
+            } else {
+                fv[q] = f.ld(t.fbase, (uint32_t)p * t.tc);
+            }
+        }
     }
 }
 
@@ -942,7 +972,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
     if (boundary) m = p.nbmask[(size_t)t.tile * t.tc + t.loc];
     R fv[Q];
     if constexpr (STAGE > 0 && S::id == LBM_STORE_F16C && MODE == MODE_DB && !BIG && L::D == 3)
-        stage_pull<L, STAGE>(t, p.g, p.f_in, fv);
+        stage_pull<L, STAGE>(t, p.g, p.f_in, boundary, fv);
     else
         load_populations<L, S, MODE, BIG>(t, fin, fv);
     constexpr bool links = LINKS && MODE == MODE_DB;
@@ -954,7 +984,10 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
             if (!links && LES != LES_MLP) return;
             active = false;
         } else {
-            if (MODE == MODE_DB) halfway_bounce_back<L, S, BIG>(t, m, fin, fv);
+            if constexpr (STAGE > 0 && S::id == LBM_STORE_F16C && MODE == MODE_DB && !BIG && L::D == 3)
+                staged_bounce_back<L, STAGE>(t, m, fin, fv);
+            else if (MODE == MODE_DB)
+                halfway_bounce_back<L, S, BIG>(t, m, fin, fv);
             uint32_t rebuilt = 0;
             if (links && (m & NB_FSB))
                 rebuilt = bouzidi_links<L, S, DDF, BIG>(t, m, fin, p.link_nodes, p.link_q, p.n_links,

@@ -84,7 +84,10 @@ typedef struct lbm_desc {
     int32_t les;               /* LBM_LES_* (config.mojo LES) */
     int32_t include_moving_boundary; /* config.mojo include_moving_boundary (reference default: 1) */
     uint32_t included_bcs;     /* bit (1u << flag enum) per BC type in the run; FLUID and SOLID are implied
-                                  (config.mojo INCLUDED_BCs) */
+                                  (config.mojo INCLUDED_BCs); LBM_FLAG_CAPTURE_DENSITY and
+                                  LBM_FLAG_CAPTURE_VELOCITY (the flag values themselves, bits 6 and 5)
+                                  turn on config.mojo capture_density / capture_velocity: a step then
+                                  writes into bc at nodes whose flag carries the bit */
     double smagorinsky_cs;     /* Cs; the reference hard-codes 0.1 (steps/collide.mojo:71) */
 } lbm_desc;
 
@@ -118,6 +121,10 @@ size_t lbm_flags_bytes(const lbm_desc* desc);
  * One double-buffer step f_in -> f_out: replaces Solver.double_buffer_step
  * (src/lbm/solver.mojo:81-111) and double_buffer_kernel
  * (src/lbm/kernels/double_buffer.mojo:25-96). f_out and f_in must not alias.
+ * With capture turned on in included_bcs, a valid node whose flag carries
+ * LBM_FLAG_CAPTURE_DENSITY / LBM_FLAG_CAPTURE_VELOCITY writes the density /
+ * velocity its collision starts from into its own bc entry (value D / values
+ * 0..D-1; steps/collide.mojo:54-63), so bc must then be writable device memory.
  */
 int lbm_step_db(lbm_handle* h, void* f_out, const void* f_in, const void* bc,
                 const uint8_t* flags, double tau, void* stream);
@@ -126,7 +133,7 @@ int lbm_step_db(lbm_handle* h, void* f_out, const void* f_in, const void* bc,
  * One in-place esoteric-pull step: replaces Solver.esoteric_pull_step[is_even]
  * (src/lbm/solver.mojo:115-154) and esoteric_pull_kernel
  * (src/lbm/kernels/esoteric_pull.mojo:24-111). Solver.even_step/odd_step pass
- * is_even = 1 / 0 (solver.mojo:159-187).
+ * is_even = 1 / 0 (solver.mojo:159-187). Captures into bc as lbm_step_db does.
  */
 int lbm_step_ep(lbm_handle* h, void* f, const void* bc, const uint8_t* flags,
                 double tau, int is_even, void* stream);

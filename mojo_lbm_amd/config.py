@@ -44,8 +44,6 @@ class LBM_Config:
         self.INCLUDED_BCs = {Flags.FLUID, Flags.SOLID} | BCs
         if not self.collision_op_is_valid():
             raise ValueError(f"Invalid collision operator {collision_op!r}")
-        if capture_density or capture_velocity:
-            raise NotImplementedError("capture_density / capture_velocity are not provided by this library")
 
     @property
     def les_id(self) -> int:
@@ -79,10 +77,16 @@ class LBM_Config:
 
     @property
     def included_bcs_mask(self) -> int:
+        """lbm_desc.included_bcs: bit (1 << flag) per BC type, plus the capture flag values
+        (Flags.CAPTURE_DENSITY / CAPTURE_VELOCITY) when capture_density / capture_velocity are on."""
         m = 0
         for b in self.INCLUDED_BCs:
             if 0 <= b < 32:
                 m |= 1 << b
+        if self.capture_density:
+            m |= Flags.CAPTURE_DENSITY
+        if self.capture_velocity:
+            m |= Flags.CAPTURE_VELOCITY
         return m
 
 

@@ -331,7 +331,7 @@ int lbm_set_exterior_wall(lbm_handle* hh, uint8_t* flags, void* bc, const char* 
     const int D = h.d.D;
     if (!flags || !bc) return set_error(LBM_ERR_INVALID, "NULL buffer");
     const bool flag_ok = flag == LBM_FLAG_FLUID || flag == LBM_FLAG_SOLID ||
-                         (flag >= 0 && flag < 32 && ((h.d.included_bcs >> flag) & 1u));
+                         (flag >= 0 && flag <= LBM_FLAG_FLUID_SOLID_BOUNDARY && ((h.d.included_bcs >> flag) & 1u));
     if (!flag_ok) return set_error(LBM_ERR_INVALID, "Input Boundary Type is not in the config's valid boundary types");
     if (!side || std::strlen(side) != 2 || (side[0] != '-' && side[0] != '+') ||
         (side[1] != 'X' && side[1] != 'Y' && side[1] != 'Z'))

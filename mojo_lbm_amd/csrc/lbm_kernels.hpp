@@ -37,6 +37,7 @@ template <class R> struct StepParams {
     R cs;
     int moving;                // include_moving_boundary
     int eqbc;                  // EQUILIBRIUM in the included BCs
+    int capture;               // LBM_FLAG_CAPTURE_DENSITY / _VELOCITY bits enabled (the step writes bc)
     uint32_t f_bytes;          // bytes of one f buffer (32-bit addressing only)
     const uint32_t* link_nodes;// interpolated bounce-back records (sorted flag-layout node indices), or NULL
     const float* link_q;       // (Q-1) link distances per record
@@ -102,6 +103,7 @@ constexpr uint32_t NB_EQ = 1u << 27;      // EQUILIBRIUM node
 constexpr uint32_t NB_FSB = 1u << 28;     // FLUID_SOLID_BOUNDARY node (interpolated links)
 constexpr uint32_t NB_DECOR = 1u << 29;   // a pull neighbour is SOLID with more flag bits set
 constexpr uint32_t NB_MOVING = 1u << 30;  // a SOLID pull neighbour is not a plain resting wall
+constexpr uint32_t NB_CAPTURE = 1u << 31; // the node's flag carries CAPTURE_DENSITY / CAPTURE_VELOCITY
 
 // ------------------------------------------------------------ population access
 // Cache policy of population stores: bit 1 (nt) streams them past the caches;
@@ -374,6 +376,7 @@ __device__ __forceinline__ uint32_t lds_mask(const TileCtx<L, I>& t, const uint8
     uint32_t m = node_valid(t.own) ? 0u : NB_INVALID;
     if ((t.own & 0x0F) == LBM_FLAG_EQUILIBRIUM) m |= NB_EQ;
     if ((t.own & 0x0F) == LBM_FLAG_FLUID_SOLID_BOUNDARY) m |= NB_FSB;
+    if (t.own & (LBM_FLAG_CAPTURE_DENSITY | LBM_FLAG_CAPTURE_VELOCITY)) m |= NB_CAPTURE;
 #pragma unroll
     for (int q = 1; q < L::Q; ++q) {
         const uint8_t nf = t.pull_flag(sfl, q);
@@ -954,6 +957,31 @@ __global__ void __launch_bounds__(64) mask_done_kernel(uint32_t* __restrict__ ch
 // double_buffer_kernel (src/lbm/kernels/double_buffer.mojo:25-96) and
 // esoteric_pull_kernel[is_even] (src/lbm/kernels/esoteric_pull.mojo:24-111).
 //
+// capture_density / capture_velocity (src/lbm/kernels/steps/collide.mojo:54-63): a node whose flag
+// carries LBM_FLAG_CAPTURE_DENSITY / _VELOCITY writes the density / velocity its collision starts
+// from (after streaming and the boundary conditions) into its own bc entry, value D / values 0..D-1.
+// The reference addresses that entry through get_rank4_coord, whose index is broken
+// (src/lbm/kernels/utils/index.mojo:120-121); the node's own entry is what it names. Only valid
+// (non-SOLID) nodes collide and capture, and no other node reads a valid node's bc in the step, so
+// the writes race with nothing.
+template <class L, class S, bool DDF, bool BIG>
+__device__ __forceinline__ void capture_moments(const TileCtx<L, Idx<BIG>>& t, const StepParams<typename S::R>& p,
+                                                const typename S::R (&fv)[L::Q]) {
+    using R = typename S::R;
+    constexpr int D = L::D;
+    const uint8_t fl = p.flags[t.field_index(p.g, t.x, t.y, t.z, 1, 0)];
+    R* bcw = const_cast<R*>(p.bc);
+    const R rho = density<L, R, DDF>(fv);
+    if ((p.capture & LBM_FLAG_CAPTURE_DENSITY) && (fl & LBM_FLAG_CAPTURE_DENSITY))
+        bcw[t.field_index(p.g, t.x, t.y, t.z, D + 1, D)] = rho;
+    if ((p.capture & LBM_FLAG_CAPTURE_VELOCITY) && (fl & LBM_FLAG_CAPTURE_VELOCITY)) {
+        R u[3];
+        velocity<L, R>(fv, rho, u);
+#pragma unroll
+        for (int d = 0; d < D; ++d) bcw[t.field_index(p.g, t.x, t.y, t.z, D + 1, d)] = u[d];
+    }
+}
+
 // step_tile updates one tile; force_slot numbers its per-wave force records. LINKS
 // compiles the interpolated bounce-back path in (double buffer only): kept out of the
 // plain kernels, whose register budget it would otherwise set.
@@ -996,6 +1024,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
                 t, p.g, m, p.flags, fin, p.bc, p.moving, p.eqbc, fv, rebuilt);
         }
     }
+    if (p.capture && (m & NB_CAPTURE) && active) capture_moments<L, S, DDF, BIG>(t, p, fv);
     if (LES == LES_MLP) {
         collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs, p.mlp);  // whole wave (matrix cores)
         if (active) store_populations<L, S, MODE, BIG>(t, fout, fv);

@@ -128,6 +128,7 @@ template <class L, class S> struct LaunchImpl {
         p.cs = (R)h.d.smagorinsky_cs;
         p.moving = h.d.include_moving_boundary != 0;
         p.eqbc = (h.d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u;
+        p.capture = (int)(h.d.included_bcs & (LBM_FLAG_CAPTURE_DENSITY | LBM_FLAG_CAPTURE_VELOCITY));
         p.f_bytes = (uint32_t)(h.big ? 0 : h.f_bytes);
         p.link_nodes = h.n_links ? h.link_nodes : nullptr;
         p.link_q = h.link_q;

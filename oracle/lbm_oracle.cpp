@@ -643,6 +643,27 @@ template <class S, class T> struct Stepper {
 
     bool valid(uint8_t flag) const { return !(flag_is_solid(flag) || flag_excluded(flag)); }  // is_valid_thread
     bool eq_enabled() const { return (d.included_bcs >> LBM_FLAG_EQUILIBRIUM) & 1u; }
+    uint32_t capture_bits() const {
+        return d.included_bcs & (LBM_FLAG_CAPTURE_DENSITY | LBM_FLAG_CAPTURE_VELOCITY);
+    }
+
+    /* capture_density / capture_velocity, src/lbm/kernels/steps/collide.mojo:54-63: the density and
+     * velocity the collision starts from (get_density / get_velocity after streaming and the
+     * boundary conditions) stored into the node's own bc entry (value D / values 0..D-1), for the
+     * flag bits the run turned on. The reference addresses the entry through get_rank4_coord, whose
+     * index is broken (src/lbm/kernels/utils/index.mojo:120-121); the node's own entry is the one it
+     * names. */
+    void capture(const T* fv, uint8_t flag, T* bc, const int idx[3]) const {
+        const int D = L.D;
+        const T rho = cell.density(fv, d.ddf_shift != 0);
+        if ((capture_bits() & LBM_FLAG_CAPTURE_DENSITY) && (flag & LBM_FLAG_CAPTURE_DENSITY))
+            bc[g.addr(idx[0], idx[1], idx[2], D, D + 1)] = rho;
+        if ((capture_bits() & LBM_FLAG_CAPTURE_VELOCITY) && (flag & LBM_FLAG_CAPTURE_VELOCITY)) {
+            T u[3] = {0, 0, 0};
+            cell.velocity(fv, rho, u);
+            for (int dd = 0; dd < D; ++dd) bc[g.addr(idx[0], idx[1], idx[2], dd, D + 1)] = u[dd];
+        }
+    }
 
     /* stream + apply_boundary_conditions for one node; returns false for invalid nodes. */
     void gather(const S* f, const T* bc, const uint8_t* flags, const int idx[3], int mode, T* fv,
@@ -688,6 +709,7 @@ int step_impl(const lbm_desc* d, S* f_out, const S* f_in, const T* bc, const uin
         T fv[27];
         uint8_t pf[27];
         st.gather(f_in, bc, flags, idx, mode, fv, pf, force_out ? &nforce[(size_t)i * 3] : nullptr);
+        if (st.capture_bits()) st.capture(fv, flag, const_cast<T*>(bc), idx);  // bc is the caller's buffer
         st.collide(fv, tau_t);
         if (mode == 0) {
             for (int q = 0; q < st.L.Q; ++q) Store<S, T>::store(f_out, g.addr(idx[0], idx[1], idx[2], q, st.K()), fv[q]);

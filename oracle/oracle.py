@@ -153,6 +153,7 @@ def mlp_closure(params, X):
 
 
 def step_db(desc, f_out, f_in, bc, flags, tau):
+    """One double-buffer step; with capture turned on in desc.included_bcs it writes bc in place."""
     _ok(lib().orc_step_db(C.byref(desc), _p(f_out), _p(f_in), _p(bc), _p(flags), float(tau)), "step_db")
 
 

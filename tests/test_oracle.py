@@ -514,3 +514,50 @@ def test_ldc_d2q9_re100_matches_ghia():
     iv = np.interp(gv[1:-1, 0], pos[1:-1], vc[1:-1])
     assert np.max(np.abs(iu - gu[1:-1, 1])) < 0.05
     assert np.max(np.abs(iv - gv[1:-1, 1])) < 0.05
+
+
+@pytest.mark.parametrize("ddf", [0, 1])
+@pytest.mark.parametrize("lat", ["D2Q9", "D3Q19", "D3Q27"])
+def test_capture_writes_pre_collision_moments(lat, ddf):
+    """capture_density / capture_velocity (steps/collide.mojo:54-63): in a periodic FP64 box the
+    populations a node collides are f_in(x - c_q, q), so the captured density and velocity equal
+    numpy's moments of those rolled values; only flagged nodes write, only the enabled kind, and
+    every other bc entry keeps its value. A capture of post-collision moments would agree here up to
+    rounding (SRT conserves both), a missing DDF +1, a wrong component order or a capture at an
+    unflagged node would not."""
+    n = (32, 16, 1) if lat == "D2Q9" else (8, 8, 8)
+    D = 2 if lat == "D2Q9" else 3
+    for mode in ("density", "velocity", "both", "off"):
+        desc = U.make_desc(lat, n, storage=2, ddf=ddf)
+        bits = {"density": 0x40, "velocity": 0x20, "both": 0x60, "off": 0}[mode]
+        desc.included_bcs |= bits
+        f, bc, flags = O.alloc(desc)
+        rng = np.random.default_rng(5 + ddf)
+        f[:] = rng.uniform(-0.01, 0.05, f.size) if ddf else rng.uniform(0.02, 0.2, f.size)
+        bc[:] = -7.0
+        ids1 = U.index_map(desc, 1)[..., 0]
+        marks = rng.integers(0, 4, n)  # 0 none, 1 density, 2 velocity, 3 both
+        flag_of = np.array([0, 0x40, 0x20, 0x60], np.uint8)
+        flags[ids1] = flag_of[marks]
+        out = f.copy()
+        O.step_db(desc, out, f, bc, flags, 0.8)
+        c, _, _ = O.lattice(D, desc.Q)
+        idsq = U.index_map(desc, desc.Q)
+        fin = f[idsq]  # (nx, ny, nz, Q)
+        cc = np.zeros((desc.Q, 3), int)
+        cc[:, :D] = np.array(c)[:, :D]
+        post = np.stack([np.roll(fin[..., q], shift=tuple(cc[q]), axis=(0, 1, 2)) for q in range(desc.Q)], -1)
+        rho = post.sum(-1) + (1.0 if ddf else 0.0)
+        u = np.einsum("xyzq,qd->xyzd", post, cc[:, :D].astype(float)) / rho[..., None]
+        idsb = U.index_map(desc, D + 1)
+        got = bc[idsb]
+        want = np.full(got.shape, -7.0)
+        if bits & 0x40:
+            sel = (marks & 1) == 1
+            want[..., D][sel] = rho[sel]
+        if bits & 0x20:
+            sel = (marks & 2) == 2
+            want[..., :D][sel] = u[sel]
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-15), mode
+        if mode == "both":
+            assert np.any(got != -7.0)

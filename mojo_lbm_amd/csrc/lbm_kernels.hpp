@@ -471,16 +471,22 @@ __device__ __forceinline__ void load_populations(const TileCtx<L, Idx<BIG>>& t, 
 // Plane q's box: for every layer and y row of the slab moved by -c_q (the pull sources, from the
 // neighbour tile where that leaves the tile), the 16-byte row of each tile column the pulls reach
 // (one for c_qx = 0; for c_qx = 1 also the column below, for -1 the one above). For c_qz != 0 the
-// box has one more layer, the slab's own layer the move leaves out, staged only in boundary tiles:
-// there the halfway bounce-back reads the node's own population of plane q from the box whenever
-// its row is in it (all but the edge row for c_qy != 0) instead of from memory. Rows in (layer,
-// tile column, row) order, planes one after the other.
+// box has one more layer after those, the slab's own layer the move leaves out, staged only in
+// boundary tiles (interior tiles skip chunks that hold nothing else): there the halfway bounce-back
+// reads the node's own population of plane q from the box whenever its row is in it (all but the
+// edge row for c_qy != 0) instead of from memory. Rows in (layer, tile column, row) order, planes
+// one after the other.
 template <class L, int BTX = 1> struct StagedBox {
     static constexpr int SZ = 4;  // slab layers
     static constexpr int ntx(int q) { return BTX + (L::c(q, 0) != 0); }
     static constexpr int nl(int q) { return SZ + (L::c(q, 2) != 0); }           // layers
-    static constexpr int llo(int q) { return L::c(q, 2) == 1 ? -1 : 0; }        // first source layer
     static constexpr int rows(int q) { return nl(q) * ntx(q) * 8; }             // 16-byte rows
+    static constexpr int pull_rows(int q) { return SZ * ntx(q) * 8; }           // rows of the pulls
+    // source layer (relative to the slab) of box layer l: the pulls' layers l - c_qz, then the
+    // slab's own layer they leave out
+    static constexpr int src_layer(int q, int l) {
+        return l < SZ ? l - L::c(q, 2) : (L::c(q, 2) == 1 ? SZ - 1 : 0);
+    }
     static constexpr int offset(int q) {  // bytes
         int s = 0;
         for (int p = 0; p < q; ++p) s += rows(p) * 16;
@@ -519,14 +525,14 @@ __device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void
         const int nch = (B::rows(q) + 63) >> 6;
 #pragma unroll
         for (int i = 0; i < nch; ++i, ++chunk) {
-            if (chunk % NW != wave) continue;  // wave-uniform
+            if (chunk % NW != wave) continue;                        // wave-uniform
+            if (!boundary && i * 64 >= B::pull_rows(q)) continue;    // own-layer rows only
             const int j = i * 64 + lane;
             const int r = j & 7, blk = j >> 3;
             const int tx = blk % B::ntx(q), l = blk / B::ntx(q);
-            const int sy = r - cy;         // source row, -1 .. 8
-            const int sl = l + B::llo(q);  // source layer relative to the slab, -1 .. SZ
-            const bool pulled = sl >= -cz && sl <= SZ - 1 - cz;
-            if (j < B::rows(q) && (boundary || pulled)) {
+            const int sy = r - cy;  // source row, -1 .. 8
+            const int sl = l < SZ ? l - cz : (cz == 1 ? SZ - 1 : 0);  // source layer in the slab, -1 .. SZ
+            if (j < B::rows(q) && (boundary || l < SZ)) {
                 const int yt = sy < 0 ? by_m : (sy > 7 ? by_p : t.by);
                 int xt = bx0 + tx;
                 xt -= xt >= g.Tx ? g.Tx : 0;
@@ -548,7 +554,7 @@ __device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void
     for (int q = 0; q < Q; ++q) {
         const int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
         const int xb = 8 * k + lx - cx + (cx == 1 ? 8 : 0);
-        const int row = ((lz - cz - B::llo(q)) * B::ntx(q) + (xb >> 3)) * 8 + ly;
+        const int row = (lz * B::ntx(q) + (xb >> 3)) * 8 + ly;
         const int16_t code = *(const int16_t*)(staged_box + B::offset(q) + ((row * 8 + (xb & 7)) << 1));
         fv[q] = f16c_decode_sext(code);
     }
@@ -568,7 +574,8 @@ __device__ __forceinline__ void staged_bounce_back(const TC& t, uint32_t m, cons
             const int ro = ly + L::c(p, 1);  // box row of the node's own row
             if (ro >= 0 && ro < 8) {
                 const int col = k + (L::c(p, 0) == 1 ? 1 : 0);
-                const int row = ((lz - B::llo(p)) * B::ntx(p) + col) * 8 + ro;
+                const int lo = lz + L::c(p, 2);  // box layer of the node's own layer
+                const int row = ((lo >= 0 && lo < B::SZ ? lo : B::SZ) * B::ntx(p) + col) * 8 + ro;
                 fv[q] = f16c_decode_sext(*(const int16_t*)(staged_box + B::offset(p) + ((row * 8 + lx) << 1)));
             } else {
                 fv[q] = f.ld(t.fbase, (uint32_t)p * t.tc);
@@ -1022,9 +1029,9 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
                                                         t.tile * t.tc + t.loc, fv, force, p.node_force);
             apply_boundaries<L, S, MODE, DDF, BIG, S::id == LBM_STORE_F32 && LES != LES_MLP>(
                 t, p.g, m, p.flags, fin, p.bc, p.moving, p.eqbc, fv, rebuilt);
+            if (p.capture && (m & NB_CAPTURE)) capture_moments<L, S, DDF, BIG>(t, p, fv);
         }
     }
-    if (p.capture && (m & NB_CAPTURE) && active) capture_moments<L, S, DDF, BIG>(t, p, fv);
     if (LES == LES_MLP) {
         collide<L, R, COLL, LES, DDF>(fv, p.tau, p.inv_tau, p.cs, p.mlp);  // whole wave (matrix cores)
         if (active) store_populations<L, S, MODE, BIG>(t, fout, fv);

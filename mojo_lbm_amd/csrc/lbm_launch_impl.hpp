@@ -152,6 +152,16 @@ template <class L, class S> struct LaunchImpl {
                                    : KIND == MK_QCRIT ? (size_t)h.g.tile_cells * sizeof(R) : 0;
         const size_t lds = p.stage_offset + stage_bytes;
         if (lds > 65536) return set_error(LBM_ERR_UNSUPPORTED, "tile too large for the output kernel's LDS (64 KiB)");
+        if constexpr (KIND == MK_FIELDS && L::D == 3) {
+            // whole-line output columns (8 x 2 x 32 nodes) on 8 x 8 x 8 tiles with Tz a multiple of 4
+            if (p.tile_kind && p.nbmask && h.g.txb == 3 && h.g.tyb == 3 && h.g.tzb == 3 && h.g.Tz % 4 == 0) {
+                if (h.big)
+                    hipLaunchKernelGGL((macro_fields_col<L, S, MODE, DDF, true>), dim3(h.g.n_tiles), dim3(512), 0, st, p);
+                else
+                    hipLaunchKernelGGL((macro_fields_col<L, S, MODE, DDF, false>), dim3(h.g.n_tiles), dim3(512), 0, st, p);
+                return check_launch("macro_fields_col");
+            }
+        }
         if (h.big)
             hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, KIND, true>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
                                lds, st, p);
@@ -178,6 +188,7 @@ template <class L, class S> struct LaunchImpl {
         if (!((uintptr_t)flags & 15u)) {
             *rc = ensure_summary(h, flags, bc, st);
             p.tile_kind = h.tile_kind;
+            p.nbmask = h.nbmask;
         }
         p.f = f;
         p.bc = (const R*)bc;

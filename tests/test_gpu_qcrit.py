@@ -127,3 +127,30 @@ def test_output_request(gpu, method_name):
     assert np.array_equal(lbm.calculate_Q_criterion(solver, asm, u, tau, True), q)
     with pytest.raises(ValueError):
         lbm.OutputRequest(solver, velocity=False, Q_criterion=True)
+
+
+COLUMN_CASES = [("D3Q19", 0, 0, 0, (32, 32, 32), False), ("D3Q19", 1, 1, 1, (32, 16, 64), False),
+                ("D3Q27", 2, 0, 0, (16, 16, 32), False), ("D3Q19", 0, 0, 1, (16, 24, 32), True),
+                ("D3Q27", 1, 1, 0, (16, 16, 32), True)]
+
+
+@pytest.mark.parametrize("lat,storage,ddf,method,n,walls", COLUMN_CASES)
+@pytest.mark.parametrize("u_layout", [0, 1])
+def test_q_criterion_tall_grids(gpu, lat, storage, ddf, method, n, walls, u_layout):
+    """Q on grids of a multiple of 4 tiles in z (the shape the fields output takes in columns):
+    bit-identical to the oracle in periodic boxes (with the domain-edge one-sided differences) and
+    in the walled box, both u layouts."""
+    print(f"case qcrit tall grid {lat} storage={storage} ddf={ddf} method={method} n={n} walls={walls} "
+          f"u_layout={u_layout}", flush=True)
+    desc = U.make_desc(lat, n, storage=storage, ddf=ddf, method=method)
+    f, bc, flags = U.perturbed_state(desc, seed=5 * storage + ddf + 2 * method, amp=2e-3, walls=walls)
+    tau = 0.58
+    run = U.DeviceRun(desc, f, bc, flags)
+    k = 3
+    _steps(run, desc, tau, k)
+    after_odd = (k % 2 == 1)
+    _, u = run.macroscopic(after_odd=after_odd, u_layout=u_layout)
+    q = run.q_criterion(u, tau, after_odd=after_odd, u_layout=u_layout)
+    oq = O.q_criterion(desc, u, run.f_host(), bc, flags, tau, after_odd=after_odd, u_layout=u_layout)
+    assert np.array_equal(q.view(np.uint8), oq.view(np.uint8)), float(np.max(np.abs(q - oq)))
+    assert np.max(np.abs(q)) > 0

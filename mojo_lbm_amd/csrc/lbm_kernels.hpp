@@ -1329,113 +1329,103 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
 }
 
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125) in columns: one 512-lane
-// workgroup per block of 8 x 2 x 32 nodes (an x row of a tile, a pair of its y rows, four tiles in
-// z; 8 x 8 x 8 tiles, Tz a multiple of 4), so the row-major output (z fastest) leaves in whole
-// 128-byte lines (32 values of rho per (x, y), 96 of u) instead of the 32-byte pieces of one tile,
-// whose lines three other tiles 1,024 tiles away in the walk complete. The four y pairs of a column
-// are consecutive units, so they share the column's population lines in L2. A column with a
-// boundary tile (by the tile kinds) is done tile by tile instead: its four workgroups take one tile
-// each, as macro_kernel does, because in a mixed column the waves of the interior tiles would idle
-// at the output barrier while the boundary waves apply the walls. Boundary tiles read their nodes'
-// neighbour masks (the step's, with the moving-wall term taken for every SOLID neighbour, as the
-// tile kernel's flag tile gives) and flags. Same arithmetic as macro_kernel, node by node.
+// workgroup per column of four z-adjacent 8 x 8 x 8 tiles (Tz a multiple of 4), each lane one node
+// of each tile in turn, at the same place in the tile. A wave reads an 8 x 8 layer of a tile per
+// population load (whole 256-byte runs), and the column's rho and u wait in LDS until all four
+// tiles are done, then leave row-major (z fastest) in whole 128-byte lines: 32 values of rho per
+// (x, y), 96 of u, one 16-byte store per lane and value group. A tile alone would leave 32-byte
+// pieces whose lines three other tiles, 1,024 tiles away in the walk, complete. Boundary tiles
+// (by the tile kinds, uniform per tile) read their nodes' neighbour masks and flags; every wave
+// takes the same four tiles, so a boundary tile delays all of the workgroup's waves alike. The
+// masks are the step's, built from the same flags and bc by the same call, so the moving-wall term
+// is read where the step reads it (NB_MOVING) and skipped next to resting walls, where it is a
+// signed zero: it could change only the sign of a zero population, and of a moment that sums to an
+// exact zero from such populations. Otherwise the same arithmetic as macro_kernel, node by node.
+// The output pointers are 16-byte aligned (checked on the host).
 template <class L, class S, int MODE, bool DDF, bool BIG>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8)))
     macro_fields_col(MacroParams<typename S::R> p) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
-    static_assert(D == 3, "columns of 3-D tiles");
-    __shared__ R srho[512];
-    __shared__ R su[512 * 3];
-    const uint32_t n_units = (uint32_t)p.g.n_tiles;  // 4 units per column of 4 tiles
-    uint32_t b = blockIdx.x;
-    if ((n_units & 7u) == 0) b = (b & 7u) * (n_units >> 3) + (b >> 3);
-    const int yp = (int)(b & 3u);
-    const uint32_t c = b >> 2;
+    static_assert(D == 3 && sizeof(R) == 4, "columns of 3-D tiles, 4-byte moments");
+    constexpr int ZS = 65;  // LDS stride of a z layer (64 (x, y) + 1: no bank conflicts either way)
+    __shared__ R srho[32 * ZS];
+    __shared__ R su[3][32 * ZS];
+    const uint32_t n_units = (uint32_t)p.g.n_tiles >> 2;  // columns
+    uint32_t c = blockIdx.x;
+    if ((n_units & 7u) == 0) c = (c & 7u) * (n_units >> 3) + (c >> 3);
     const uint32_t Tx = (uint32_t)p.g.Tx, Ty = (uint32_t)p.g.Ty;
     const int bx = (int)(c % Tx), by = (int)((c / Tx) % Ty), bz4 = (int)(c / (Tx * Ty));
     const uint32_t t0 = ((uint32_t)(bz4 * 4) * Ty + (uint32_t)by) * Tx + (uint32_t)bx, tstride = Tx * Ty;
-    const bool column = ((p.tile_kind[t0] | p.tile_kind[t0 + tstride] | p.tile_kind[t0 + 2 * tstride] |
-                          p.tile_kind[t0 + 3 * tstride]) & 1u) == 0u;  // uniform
-    const int tid = threadIdx.x;
-    int lx, ly, lz, slot;
-    uint32_t tile;
-    if (column) {  // 8 x 2 x 32: lane x + 8 (y + 2 z)
-        lx = tid & 7;
-        ly = yp * 2 + ((tid >> 3) & 1);
-        const int cz = tid >> 4;
-        tile = t0 + (uint32_t)(cz >> 3) * tstride;
-        lz = cz & 7;
-        slot = (lx * 2 + ((tid >> 3) & 1)) * 32 + cz;
-    } else {  // tile yp of the column, 8 x 8 x 8
-        lx = tid & 7;
-        ly = (tid >> 3) & 7;
-        lz = tid >> 6;
-        tile = t0 + (uint32_t)yp * tstride;
-        slot = (lx * 8 + ly) * 8 + lz;
-    }
-    TileCtx<L, Idx<BIG>> t;
-    t.init_pos(p.g, tile, 0, p.g.tzb, lx + 8 * ly + 64 * lz);
+    const int tid = threadIdx.x, lxy = tid & 63, lz = tid >> 6;
     const Field<S, BIG> f(p.f, p.f_bytes);
-    const bool boundary = (p.tile_kind[t.tile] & 1u) != 0u;  // uniform per wave
-    uint32_t m = 0;
-    uint8_t own = LBM_FLAG_FLUID;
-    if (boundary) {
-        const size_t node = (size_t)t.tile * t.tc + t.loc;
-        m = p.nbmask[node];
-        own = p.flags[node];
-    }
-    R fv[Q];
-    load_populations<L, S, MODE, BIG>(t, f, fv);
-    if (m) {
-        m |= NB_MOVING;  // the moving-wall term read for every SOLID neighbour
-        uint32_t rebuilt = 0;
-        if (MODE == MODE_DB) {
-            halfway_bounce_back<L, S, BIG>(t, m, f, fv);
-            if (p.link_nodes && (m & NB_FSB)) {
-                double unused[3] = {0, 0, 0};
-                rebuilt = bouzidi_links<L, S, DDF, BIG>(t, m, f, p.link_nodes, p.link_q, p.n_links,
-                                                        t.tile * t.tc + t.loc, fv, unused);
-            }
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        TileCtx<L, Idx<BIG>> t;
+        t.init_pos(p.g, t0 + (uint32_t)k * tstride, 0, p.g.tzb, tid);
+        const bool boundary = (p.tile_kind[t.tile] & 1u) != 0u;  // uniform
+        uint32_t m = 0;
+        uint8_t own = LBM_FLAG_FLUID;
+        if (boundary) {
+            const size_t node = (size_t)t.tile * t.tc + t.loc;
+            m = p.nbmask[node];
+            own = p.flags[node];
         }
-        apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
+        R fv[Q];
+        load_populations<L, S, MODE, BIG>(t, f, fv);
+        if (m) {
+            uint32_t rebuilt = 0;
+            if (MODE == MODE_DB) {
+                halfway_bounce_back<L, S, BIG>(t, m, f, fv);
+                if (p.link_nodes && (m & NB_FSB)) {
+                    double unused[3] = {0, 0, 0};
+                    rebuilt = bouzidi_links<L, S, DDF, BIG>(t, m, f, p.link_nodes, p.link_q, p.n_links,
+                                                            t.tile * t.tc + t.loc, fv, unused);
+                }
+            }
+            apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
+        }
+        R rho, u[3] = {0, 0, 0};
+        if (own != LBM_FLAG_SOLID) {
+            rho = density<L, R, DDF>(fv);
+            velocity<L, R>(fv, rho, u);
+        } else {
+#pragma unroll
+            for (int d = 0; d < D; ++d) u[d] = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, d)];
+            rho = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, D)];
+        }
+        const int a = (k * 8 + lz) * ZS + lxy;
+        srho[a] = rho;
+#pragma unroll
+        for (int d = 0; d < D; ++d) su[d][a] = u[d];
     }
-    R rho, u[3] = {0, 0, 0};
-    if (own != LBM_FLAG_SOLID) {
-        rho = density<L, R, DDF>(fv);
-        velocity<L, R>(fv, rho, u);
+    __syncthreads();
+    // 64 runs of 32 z per (x, y); lane: run tid / 8, z 4 (tid % 8) .. + 3 of rho (and of each u
+    // plane), or three of the 24 16-byte pieces of the interleaved u runs
+    const size_t ny = (size_t)p.g.ny, nz = (size_t)p.g.nz, ncell = (size_t)p.g.nx * ny * nz;
+    const int x0 = bx * 8, y0 = by * 8;
+    const size_t z0 = (size_t)bz4 * 32;
+    auto row = [&](int xy) { return ((size_t)(x0 + (xy & 7)) * ny + (size_t)(y0 + (xy >> 3))) * nz + z0; };
+    auto gather4 = [&](const R* lds, int xy, int z) {
+        return make_float4(lds[z * ZS + xy], lds[(z + 1) * ZS + xy], lds[(z + 2) * ZS + xy], lds[(z + 3) * ZS + xy]);
+    };
+    const int xy = tid >> 3, z4 = (tid & 7) * 4;
+    const size_t r = row(xy) + (size_t)z4;
+    *reinterpret_cast<float4*>(p.rho + r) = gather4(srho, xy, z4);
+    if (p.u_layout != LBM_U_XYZD) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) *reinterpret_cast<float4*>(p.u + (size_t)d * ncell + r) = gather4(su[d], xy, z4);
     } else {
 #pragma unroll
-        for (int d = 0; d < D; ++d) u[d] = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, d)];
-        rho = p.bc[t.field_index(p.g, t.x, t.y, t.z, D + 1, D)];
-    }
-    const bool planar = p.u_layout != LBM_U_XYZD;
-    srho[slot] = rho;
+        for (int i = 0; i < 3; ++i) {
+            const int g = tid + 512 * i, gxy = g / 24, e = g - 24 * gxy;  // piece e of run gxy
+            float w[4];
 #pragma unroll
-    for (int d = 0; d < D; ++d) su[planar ? d * 512 + slot : slot * D + d] = u[d];
-    __syncthreads();
-    // runs of nz_run consecutive z per (x, y): 16 (x, y) x 32 z in a column, 64 (x, y) x 8 z in a tile
-    const size_t ny = (size_t)p.g.ny, nz = (size_t)p.g.nz, ncell = (size_t)p.g.nx * ny * nz;
-    const int run_bits = column ? 5 : 3, run = 1 << run_bits;
-    const int x0 = bx * 8, y0 = by * 8 + (column ? yp * 2 : 0);
-    const size_t z0 = column ? (size_t)bz4 * 32 : (size_t)(bz4 * 4 + yp) * 8;
-    auto row = [&](int xy) {  // output index of the run of (x, y) pair xy
-        const int xs = column ? (xy >> 1) : (xy >> 3), ys = column ? (xy & 1) : (xy & 7);
-        return ((size_t)(x0 + xs) * ny + (size_t)(y0 + ys)) * nz + z0;
-    };
-    {
-        const size_t r = row(tid >> run_bits) + (size_t)(tid & (run - 1));
-        p.rho[r] = srho[tid];
-        if (planar) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) p.u[(size_t)d * ncell + r] = su[d * 512 + tid];
-        }
-    }
-    if (!planar) {
-#pragma unroll
-        for (int k = tid; k < 512 * D; k += 512) {
-            const int xy = k / (run * D), e = k % (run * D);
-            p.u[row(xy) * D + e] = su[k];
+            for (int j = 0; j < 4; ++j) {
+                const int idx = e * 4 + j, z = idx / 3;
+                w[j] = su[idx - 3 * z][z * ZS + gxy];
+            }
+            *reinterpret_cast<float4*>(p.u + row(gxy) * 3 + (size_t)e * 4) = make_float4(w[0], w[1], w[2], w[3]);
         }
     }
 }

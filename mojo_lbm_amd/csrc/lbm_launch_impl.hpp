@@ -152,13 +152,17 @@ template <class L, class S> struct LaunchImpl {
                                    : KIND == MK_QCRIT ? (size_t)h.g.tile_cells * sizeof(R) : 0;
         const size_t lds = p.stage_offset + stage_bytes;
         if (lds > 65536) return set_error(LBM_ERR_UNSUPPORTED, "tile too large for the output kernel's LDS (64 KiB)");
-        if constexpr (KIND == MK_FIELDS && L::D == 3) {
-            // whole-line output columns (8 x 2 x 32 nodes) on 8 x 8 x 8 tiles with Tz a multiple of 4
-            if (p.tile_kind && p.nbmask && h.g.txb == 3 && h.g.tyb == 3 && h.g.tzb == 3 && h.g.Tz % 4 == 0) {
+        if constexpr (KIND == MK_FIELDS && L::D == 3 && sizeof(R) == 4) {
+            // whole-line output columns (four z-adjacent tiles) on 8 x 8 x 8 tiles with Tz a multiple of 4,
+            // 4-byte moments and 16-byte aligned outputs (FP64 moments take the tile kernel)
+            const bool aligned = (((uintptr_t)p.rho | (uintptr_t)p.u) & 15u) == 0;
+            if (aligned && p.tile_kind && p.nbmask && h.g.txb == 3 && h.g.tyb == 3 &&
+                h.g.tzb == 3 && h.g.Tz % 4 == 0) {
+                const dim3 grid((unsigned)(h.g.n_tiles / 4));
                 if (h.big)
-                    hipLaunchKernelGGL((macro_fields_col<L, S, MODE, DDF, true>), dim3(h.g.n_tiles), dim3(512), 0, st, p);
+                    hipLaunchKernelGGL((macro_fields_col<L, S, MODE, DDF, true>), grid, dim3(512), 0, st, p);
                 else
-                    hipLaunchKernelGGL((macro_fields_col<L, S, MODE, DDF, false>), dim3(h.g.n_tiles), dim3(512), 0, st, p);
+                    hipLaunchKernelGGL((macro_fields_col<L, S, MODE, DDF, false>), grid, dim3(512), 0, st, p);
                 return check_launch("macro_fields_col");
             }
         }

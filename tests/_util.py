@@ -95,12 +95,14 @@ class DeviceRun:
         _lib.check(self.L.lbm_step_ep(self.h.ptr, self.f.data_ptr(), self.bc.data_ptr(), self.flags.data_ptr(),
                                       float(tau), int(is_even), self.stream))
 
-    def macroscopic(self, after_odd=False, u_layout=0):
+    def macroscopic(self, after_odd=False, u_layout=0, misalign=False):
+        """rho and u on the host; misalign: output buffers one element past a 16-byte boundary."""
         t = self.torch
         dt = t.float64 if self.desc.storage == 2 else t.float32
         n = self.desc.nx * self.desc.ny * self.desc.nz
-        rho = t.empty(n, dtype=dt, device="cuda")
-        u = t.empty(n * self.desc.D, dtype=dt, device="cuda")
+        k = 1 if misalign else 0
+        rho = t.empty(n + k, dtype=dt, device="cuda")[k:]
+        u = t.empty(n * self.desc.D + k, dtype=dt, device="cuda")[k:]
         _lib.check(self.L.lbm_macroscopic(self.h.ptr, rho.data_ptr(), u.data_ptr(), self.f.data_ptr(),
                                           self.bc.data_ptr(), self.flags.data_ptr(), int(after_odd), int(u_layout),
                                           self.stream))

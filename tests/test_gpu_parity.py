@@ -237,18 +237,17 @@ def test_capture_matches_oracle(gpu, lat, store, ddf, method):
 COLUMN_CASES = [("D3Q19", "f32", 0, (16, 16, 32), False, True), ("D3Q19", "f32", 1, (24, 16, 32), False, True),
                 ("D3Q19", "f16c", 0, (16, 24, 32), True, True), ("D3Q19", "f64", 1, (16, 16, 32), False, True),
                 ("D3Q27", "f32", 0, (16, 16, 32), True, True), ("D3Q27", "f16c", 1, (8, 16, 64), False, True),
-                # periodic boxes: columns without boundary tiles take the 8 x 2 x 32 path, the columns
-                # through the sphere (double buffer) the tile path
+                # periodic boxes, with a sphere and its links in the double-buffer cases
                 ("D3Q19", "f32", 0, (32, 32, 32), False, False), ("D3Q19", "f32", 1, (32, 16, 64), False, False),
                 ("D3Q27", "f16c", 0, (32, 32, 32), False, False), ("D3Q19", "f64", 0, (16, 16, 64), False, False)]
 
 
 @pytest.mark.parametrize("lat,store,method,grid,eq,walls", COLUMN_CASES)
 def test_macroscopic_columns(gpu, lat, store, method, grid, eq, walls):
-    """The fields output in 8 x 2 x 32 columns (8x8x8 tiles, a multiple of 4 tiles in z; columns with
-    a boundary tile tile by tile): rho and u bit-identical to the oracle in both u layouts, with
-    walls, the moving lid, SOLID nodes reporting bc, an equilibrium outlet face and interpolated
-    links on a sphere (double buffer), and in periodic boxes."""
+    """The fields output in columns of four z-adjacent tiles (8x8x8 tiles, a multiple of 4 tiles in
+    z; FP64 and outputs off a 16-byte boundary take the tile kernel): rho and u bit-identical to the
+    oracle in both u layouts, with walls, the moving lid, SOLID nodes reporting bc, an equilibrium
+    outlet face and interpolated links on a sphere (double buffer), and in periodic boxes."""
     print(f"case macro columns {lat} {store} method={method} grid={grid} eq={eq} walls={walls}", flush=True)
     desc = U.make_desc(lat, grid, method=method, storage=STORES[store], ddf=1 if store == "f16c" else 0, eq=eq)
     f, bc, flags = U.perturbed_state(desc, seed=17 + method, walls=walls)
@@ -281,8 +280,9 @@ def test_macroscopic_columns(gpu, lat, store, method, grid, eq, walls):
         ref_f = U.oracle_run_ep(desc, f, bc, flags, TAU, 2)
         after_odd = True
     for layout in (0, 1):
-        rho, u = run.macroscopic(after_odd, layout)
         kw = dict(links=links) if links else {}
         rr, ur = O.macroscopic(desc, ref_f, bc, flags, after_odd, layout, **kw)
-        assert np.array_equal(rho.view(np.uint8), rr.view(np.uint8))
-        assert np.array_equal(u.view(np.uint8), ur.view(np.uint8))
+        for misalign in (False, True):
+            rho, u = run.macroscopic(after_odd, layout, misalign=misalign)
+            assert np.array_equal(rho.view(np.uint8), rr.view(np.uint8)), f"rho, layout {layout}, misalign {misalign}"
+            assert np.array_equal(u.view(np.uint8), ur.view(np.uint8)), f"u, layout {layout}, misalign {misalign}"

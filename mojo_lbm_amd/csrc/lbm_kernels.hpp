@@ -1101,28 +1101,66 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
 template <class R, int D, bool FLAGS = true>
 __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3], const R* su, const uint8_t* sfl,
                                          int center, const int (&lstride)[3], int comp, int axis) {
+    // A SOLID neighbour sits at half distance (dx 0.5, so the difference is scaled by 1 / dx = 2,
+    // exactly); the left weight is 1 - dx_l / (dx_l + dx_r). The quotients take four values,
+    // written as constants that round as the division does, so no division is executed.
+    constexpr R w_equal = (R)1 - (R)1 / ((R)1 + (R)1);          // dx_l = dx_r (1 / 2 = 0.5 / 1)
+    constexpr R w_right_half = (R)1 - (R)1 / ((R)1 + (R)0.5);   // dx_l = 1, dx_r = 0.5
+    constexpr R w_left_half = (R)1 - (R)0.5 / ((R)0.5 + (R)1);  // dx_l = 0.5, dx_r = 1
     const R uc = su[center * D + comp];
-    R left_grad = 0, left_dx = 0, right_grad = 0, right_dx = 0;
+    R left_grad = 0, right_grad = 0;
+    bool left_half = false, right_half = false;
     const bool left_ok = pos[axis] - 1 >= 0;
     if (left_ok) {
         const int c = center - lstride[axis];
-        const R ua = su[c * D + comp];
-        const R gdx = (FLAGS && sfl[c] == LBM_FLAG_SOLID) ? (R)0.5 : (R)1;
-        left_grad = (uc - ua) * ((R)1 / gdx);
-        left_dx = gdx;
+        left_half = FLAGS && sfl[c] == LBM_FLAG_SOLID;
+        left_grad = (uc - su[c * D + comp]) * (left_half ? (R)2 : (R)1);
     }
     const bool right_ok = pos[axis] + 1 < n[axis];
     if (right_ok) {
         const int c = center + lstride[axis];
-        const R ua = su[c * D + comp];
-        const R gdx = (FLAGS && sfl[c] == LBM_FLAG_SOLID) ? (R)0.5 : (R)1;
-        right_grad = (ua - uc) * ((R)1 / gdx);
-        right_dx = gdx;
+        right_half = FLAGS && sfl[c] == LBM_FLAG_SOLID;
+        right_grad = (su[c * D + comp] - uc) * (right_half ? (R)2 : (R)1);
     }
-    const R total = left_dx + right_dx;
-    const R left_w = (R)1 - left_dx / total;
+    // one side only: dx_l / dx_l = 1 (left) or 0 / dx_r = 0 (right)
+    const R left_w = !right_ok ? (R)0
+                     : !left_ok ? (R)1
+                     : left_half == right_half ? w_equal
+                     : right_half ? w_right_half : w_left_half;
     const R right_w = (R)1 - left_w;
     return (left_ok || right_ok) ? right_grad * right_w + left_grad * left_w : (R)0;
+}
+
+// Q = 0.25 |omega|^2 - 0.5 |S|^2 of a node (calculate_Q_criterion,
+// src/lbm/output/Q_criterion.mojo:90-215): omega from finite differences of the velocity box su
+// (D values per cell, strides ls, the node at center; FLAGS: SOLID neighbours in sfl, same cell
+// order, at half distance), S from the node's re-streamed populations fv.
+template <class L, class R, bool DDF, bool FLAGS>
+__device__ __forceinline__ R q_value(const int (&pos)[3], const int (&nn)[3], const R* su, const uint8_t* sfl,
+                                     int center, const int (&ls)[3], const R (&fv)[L::Q], R tau) {
+    constexpr int Q = L::Q, D = L::D;
+    auto grad = [&](int comp, int axis) { return fd_gradient<R, D, FLAGS>(pos, nn, su, sfl, center, ls, comp, axis); };
+    R wsq;
+    if (D == 2) {
+        const R w = grad(1, 0) - grad(0, 1);
+        wsq = w * w;
+    } else {
+        const R wx = grad(2, 1) - grad(1, 2);
+        const R wy = grad(0, 2) - grad(2, 0);
+        const R wz = grad(1, 0) - grad(0, 1);
+        wsq = wx * wx;
+        wsq = wsq + wy * wy;
+        wsq = wsq + wz * wz;
+    }
+    const R rho = density<L, R, DDF>(fv);
+    R u[3];
+    velocity<L, R>(fv, rho, u);
+    const R uu = dot_uu<L, R>(u);
+    R fn[Q], pi[6];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fn[q] = fv[q] - feq<L, R, DDF>(q, rho, u, uu);
+    pi_neq<L, R>(fn, pi);
+    return (R)0.25 * wsq - (R)0.5 * strain_norm_sq<L, R>(pi, rho, tau);
 }
 
 // Loads the velocities of a tile and its one-node halo (cells outside the grid: 0, never
@@ -1245,31 +1283,8 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
             const int pos[3] = {t.x, t.y, t.z};
             const int nn[3] = {p.g.nx, p.g.ny, p.g.nz};
             const int ls[3] = {1, t.hx, t.hxy};
-            auto grad = [&](int comp, int axis) {
-                return boundary ? fd_gradient<R, D, true>(pos, nn, su, sfl, t.center, ls, comp, axis)
-                                : fd_gradient<R, D, false>(pos, nn, su, sfl, t.center, ls, comp, axis);
-            };
-            R wsq;
-            if (D == 2) {
-                const R w = grad(1, 0) - grad(0, 1);
-                wsq = w * w;
-            } else {
-                const R wx = grad(2, 1) - grad(1, 2);
-                const R wy = grad(0, 2) - grad(2, 0);
-                const R wz = grad(1, 0) - grad(0, 1);
-                wsq = wx * wx;
-                wsq = wsq + wy * wy;
-                wsq = wsq + wz * wz;
-            }
-            const R rho = density<L, R, DDF>(fv);
-            R u[3];
-            velocity<L, R>(fv, rho, u);
-            const R uu = dot_uu<L, R>(u);
-            R fn[Q], pi[6];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) fn[q] = fv[q] - feq<L, R, DDF>(q, rho, u, uu);
-            pi_neq<L, R>(fn, pi);
-            qv = (R)0.25 * wsq - (R)0.5 * strain_norm_sq<L, R>(pi, rho, p.tau);
+            qv = boundary ? q_value<L, R, DDF, true>(pos, nn, su, sfl, t.center, ls, fv, p.tau)
+                          : q_value<L, R, DDF, false>(pos, nn, su, sfl, t.center, ls, fv, p.tau);
         }
         stage[slot] = qv;
         __syncthreads();
@@ -1428,6 +1443,127 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
             *reinterpret_cast<float4*>(p.u + row(gxy) * 3 + (size_t)e * 4) = make_float4(w[0], w[1], w[2], w[3]);
         }
     }
+}
+
+// calculate_Q_criterion (src/lbm/output/Q_criterion.mojo:90-215) in the columns of
+// macro_fields_col: one 512-lane workgroup per four z-adjacent 8 x 8 x 8 tiles, each lane one node
+// of each tile in turn (a wave reads an 8 x 8 layer per population load). The velocity box of the
+// column and its one-node halo (10 x 10 x 34 nodes, 1.7 velocities read per node against 2.0 for a
+// tile's 10 x 10 x 10 box) is staged in LDS first, with the flags of the same cells when a tile of
+// the column is a boundary tile; Q then waits in LDS and leaves row-major in whole 128-byte lines,
+// one 16-byte store per lane. Boundary tiles take their masks and own flags as macro_fields_col
+// does; the gradient of a boundary tile reads the flag box (SOLID neighbours at half distance),
+// that of an interior tile (halo all FLUID) none. Same arithmetic as macro_kernel (q_value).
+template <class L, class S, int MODE, bool DDF, bool BIG>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    macro_qcrit_col(MacroParams<typename S::R> p) {
+    using R = typename S::R;
+    constexpr int Q = L::Q, D = L::D;
+    static_assert(D == 3 && sizeof(R) == 4, "columns of 3-D tiles, 4-byte moments");
+    constexpr int HX = 10, HY = 10, HZ = 34, HC = HX * HY * HZ;  // velocity box: tile column + halo
+    constexpr int ZS = 65;
+    __shared__ R su[HC * D];
+    __shared__ uint8_t sfl[HC];
+    __shared__ R sq[32 * ZS];
+    const uint32_t n_units = (uint32_t)p.g.n_tiles >> 2;
+    uint32_t c = blockIdx.x;
+    if ((n_units & 7u) == 0) c = (c & 7u) * (n_units >> 3) + (c >> 3);
+    const uint32_t Tx = (uint32_t)p.g.Tx, Ty = (uint32_t)p.g.Ty;
+    const int bx = (int)(c % Tx), by = (int)((c / Tx) % Ty), bz4 = (int)(c / (Tx * Ty));
+    const uint32_t t0 = ((uint32_t)(bz4 * 4) * Ty + (uint32_t)by) * Tx + (uint32_t)bx, tstride = Tx * Ty;
+    const int tid = threadIdx.x, lxy = tid & 63, lz = tid >> 6;
+    uint32_t kinds = 0;  // bit k: tile k of the column is a boundary tile (uniform)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) kinds |= (p.tile_kind[t0 + (uint32_t)k * tstride] & 1u) << k;
+    const bool any_boundary = kinds != 0u;
+    const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
+    const size_t ncell = (size_t)nx * ny * nz;
+    const bool planar = p.u_layout != LBM_U_XYZD;
+    const int gx0 = bx * 8 - 1, gy0 = by * 8 - 1, gz0 = bz4 * 32 - 1;
+    // The boxes row by row: wave w takes rows (x, y) w, w + 8, ... of the 10 x 10; a row is the 34 z
+    // of the box, D values each (102 consecutive words in the interleaved layout, D runs of 34 in
+    // the planar one), lane j and j + 64 reading word j and j + 64 of it. The per-lane part of the
+    // addresses is fixed; the row adds a uniform offset. Cells outside the grid are never read by
+    // the gradient and are left as they are.
+    const int lane = tid & 63, wave = tid >> 6;
+    int64_t voff[2];
+    int vlds[2];
+    bool vok[2];
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+        const int j = lane + 64 * s2;
+        const int iz = planar ? j % HZ : j / D, d = planar ? j / HZ : j % D;
+        vok[s2] = j < HZ * D && gz0 + iz >= 0 && gz0 + iz < nz;
+        voff[s2] = planar ? (int64_t)d * (int64_t)ncell + iz : (int64_t)j;
+        vlds[s2] = iz * (HX * HY * D) + d;
+    }
+    for (int row = wave; row < HX * HY; row += 8) {
+        const int ix = row / HY, iy = row - ix * HY;  // uniform
+        const int gx = gx0 + ix, gy = gy0 + iy;
+        if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+        const int64_t rz = ((int64_t)gx * ny + gy) * nz + gz0;  // row-major index of (gx, gy, gz0)
+        const int64_t base = planar ? rz : rz * D;
+        const int rl = (iy * HX + ix) * D;
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2)
+            if (vok[s2]) su[vlds[s2] + rl] = p.vel[base + voff[s2]];
+    }
+    if (any_boundary) {  // flags (tiled layout) of the same cells, lane iz of each row
+        const bool zok = lane < HZ && gz0 + lane >= 0 && gz0 + lane < nz;
+        const int gz = gz0 + lane;
+        const size_t zoff = zok ? (size_t)(gz >> 3) * (size_t)(p.g.Tx * p.g.Ty) * 512u + (size_t)(64 * (gz & 7)) : 0;
+        for (int row = wave; row < HX * HY; row += 8) {
+            const int ix = row / HY, iy = row - ix * HY;
+            const int gx = gx0 + ix, gy = gy0 + iy;
+            if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+            const size_t xyoff = (size_t)((gx >> 3) + p.g.Tx * (gy >> 3)) * 512u + (size_t)((gx & 7) + 8 * (gy & 7));
+            if (zok) sfl[(lane * HY + iy) * HX + ix] = p.flags[xyoff + zoff];
+        }
+    }
+    __syncthreads();
+    const Field<S, BIG> f(p.f, p.f_bytes);
+    const int nn[3] = {nx, ny, nz};
+    const int ls[3] = {1, HX, HX * HY};
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        TileCtx<L, Idx<BIG>> t;
+        t.init_pos(p.g, t0 + (uint32_t)k * tstride, 0, p.g.tzb, tid);
+        const bool boundary = ((kinds >> k) & 1u) != 0u;  // uniform
+        uint32_t m = 0;
+        uint8_t own = LBM_FLAG_FLUID;
+        if (boundary) {
+            const size_t node = (size_t)t.tile * t.tc + t.loc;
+            m = p.nbmask[node];
+            own = p.flags[node];
+        }
+        R fv[Q];
+        load_populations<L, S, MODE, BIG>(t, f, fv);
+        if (m) {
+            uint32_t rebuilt = 0;
+            if (MODE == MODE_DB) {
+                halfway_bounce_back<L, S, BIG>(t, m, f, fv);
+                if (p.link_nodes && (m & NB_FSB)) {
+                    double unused[3] = {0, 0, 0};
+                    rebuilt = bouzidi_links<L, S, DDF, BIG>(t, m, f, p.link_nodes, p.link_q, p.n_links,
+                                                            t.tile * t.tc + t.loc, fv, unused);
+                }
+            }
+            apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
+        }
+        R qv = 0;
+        if (own != LBM_FLAG_SOLID) {
+            const int pos[3] = {t.x, t.y, t.z};
+            const int center = ((k * 8 + lz + 1) * HY + (tid >> 3 & 7) + 1) * HX + (tid & 7) + 1;
+            qv = boundary ? q_value<L, R, DDF, true>(pos, nn, su, sfl, center, ls, fv, p.tau)
+                          : q_value<L, R, DDF, false>(pos, nn, su, sfl, center, ls, fv, p.tau);
+        }
+        sq[(k * 8 + lz) * ZS + lxy] = qv;
+    }
+    __syncthreads();
+    const int xy = tid >> 3, z4 = (tid & 7) * 4;
+    const size_t r = ((size_t)(bx * 8 + (xy & 7)) * ny + (size_t)(by * 8 + (xy >> 3))) * nz + (size_t)(bz4 * 32 + z4);
+    *reinterpret_cast<float4*>(p.qout + r) =
+        make_float4(sq[z4 * ZS + xy], sq[(z4 + 1) * ZS + xy], sq[(z4 + 2) * ZS + xy], sq[(z4 + 3) * ZS + xy]);
 }
 
 // Sums the per-tile partials in tile order (one workgroup, lane-strided, then lane order).

@@ -166,6 +166,18 @@ template <class L, class S> struct LaunchImpl {
                 return check_launch("macro_fields_col");
             }
         }
+        if constexpr (KIND == MK_QCRIT && L::D == 3 && sizeof(R) == 4) {
+            // the same columns for the Q-criterion (16-byte aligned output)
+            if (!((uintptr_t)p.qout & 15u) && p.tile_kind && p.nbmask && h.g.txb == 3 && h.g.tyb == 3 &&
+                h.g.tzb == 3 && h.g.Tz % 4 == 0) {
+                const dim3 grid((unsigned)(h.g.n_tiles / 4));
+                if (h.big)
+                    hipLaunchKernelGGL((macro_qcrit_col<L, S, MODE, DDF, true>), grid, dim3(512), 0, st, p);
+                else
+                    hipLaunchKernelGGL((macro_qcrit_col<L, S, MODE, DDF, false>), grid, dim3(512), 0, st, p);
+                return check_launch("macro_qcrit_col");
+            }
+        }
         if (h.big)
             hipLaunchKernelGGL((macro_kernel<L, S, MODE, DDF, KIND, true>), dim3(h.g.n_tiles), dim3(h.g.tile_cells),
                                lds, st, p);

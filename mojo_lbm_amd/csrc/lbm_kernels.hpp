@@ -1367,6 +1367,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     __shared__ R srho[32 * ZS];
     __shared__ R su[3][32 * ZS];
     const uint32_t n_units = (uint32_t)p.g.n_tiles >> 2;  // columns
+    if (blockIdx.x >= n_units) return;                     // a launch of more workgroups than columns
     uint32_t c = blockIdx.x;
     if ((n_units & 7u) == 0) c = (c & 7u) * (n_units >> 3) + (c >> 3);
     const uint32_t Tx = (uint32_t)p.g.Tx, Ty = (uint32_t)p.g.Ty;
@@ -1465,7 +1466,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
     __shared__ R su[HC * D];
     __shared__ uint8_t sfl[HC];
     __shared__ R sq[32 * ZS];
-    const uint32_t n_units = (uint32_t)p.g.n_tiles >> 2;
+    const uint32_t n_units = (uint32_t)p.g.n_tiles >> 2;  // columns
+    if (blockIdx.x >= n_units) return;                     // a launch of more workgroups than columns
     uint32_t c = blockIdx.x;
     if ((n_units & 7u) == 0) c = (c & 7u) * (n_units >> 3) + (c >> 3);
     const uint32_t Tx = (uint32_t)p.g.Tx, Ty = (uint32_t)p.g.Ty;

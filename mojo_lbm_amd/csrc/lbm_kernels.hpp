@@ -504,7 +504,8 @@ extern __shared__ __attribute__((aligned(16))) uint8_t staged_box[];
 // 16 bytes at (chunk address + 16 j), so row order in LDS is lane order. BTX = 2: the workgroup
 // covers two x-adjacent tiles (waves 0-3 the first, 4-7 the second), whose boxes share their rows.
 template <class L, int BTX, class TC, class FV>
-__device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void* f_in, bool boundary, FV& fv) {
+__device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void* f_in, uint32_t f_bytes, bool boundary,
+                                           FV& fv) {
     using B = StagedBox<L, BTX>;
     constexpr int Q = L::Q, NW = 4 * BTX, SZ = B::SZ;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
@@ -516,6 +517,56 @@ __device__ __forceinline__ void stage_pull(const TC& t, const Geo& g, const void
     // tile rows below / above (periodic)
     const int by_m = t.by > 0 ? t.by - 1 : g.Ty - 1, by_p = t.by < g.Ty - 1 ? t.by + 1 : 0;
     int chunk = 0;  // chunks of the planes before q
+    if constexpr (BTX == 1) {
+        // One tile per workgroup: a row's source tile is the tile itself or a neighbour (periodic)
+        // in x, y and z, whose element bases are uniform; each lane picks among them by its row
+        // (r = lane % 8), its layer and its tile column (b = lane / 8 gives both), and the plane's
+        // offset goes in the scalar offset of a buffer load straight to LDS.
+        const int r = lane & 7, b = lane >> 3;
+        const uint32_t TxTQ = (uint32_t)g.Tx * TQ, TxyTQ = (uint32_t)g.Ty * TxTQ;
+        const int bx_m = t.bx > 0 ? t.bx - 1 : g.Tx - 1, bx_p = t.bx < g.Tx - 1 ? t.bx + 1 : 0;
+        const int bz_m = t.bz > 0 ? t.bz - 1 : g.Tz - 1, bz_p = t.bz < g.Tz - 1 ? t.bz + 1 : 0;
+        // (read as wave-uniform values, so that a lane's choice selects among the products instead
+        // of multiplying a selected factor)
+        auto uni = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); };
+        const uint32_t Y[3] = {uni((uint32_t)by_m * TxTQ), uni((uint32_t)t.by * TxTQ), uni((uint32_t)by_p * TxTQ)};
+        const uint32_t Z[3] = {uni((uint32_t)bz_m * TxyTQ), uni((uint32_t)t.bz * TxyTQ), uni((uint32_t)bz_p * TxyTQ)};
+        const uint32_t X[3] = {uni((uint32_t)bx_m * TQ), uni((uint32_t)t.bx * TQ), uni((uint32_t)bx_p * TQ)};
+        uint32_t yo[3];  // source row r - c_y (c_y = -1, 0, 1): its tile row and row in the tile
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int sy = r - (d - 1);
+            yo[d] = (sy < 0 ? Y[0] : sy > 7 ? Y[2] : Y[1]) + ((uint32_t)(sy & 7) << 3);
+        }
+        // tile column of a two-column box (c_x = -1: columns bx, bx + 1; c_x = 1: bx - 1, bx)
+        const bool second = (b & 1) != 0;
+        const uint32_t xo_neg = second ? X[2] : X[1], xo_pos = second ? X[1] : X[0];
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)f_in, 0, (int)f_bytes, 0x00020000);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
+            const int nch = (B::rows(q) + 63) >> 6;
+#pragma unroll
+            for (int i = 0; i < nch; ++i, ++chunk) {
+                if (chunk % NW != wave) continue;                        // wave-uniform
+                if (!boundary && i * 64 >= B::pull_rows(q)) continue;    // own-layer rows only
+                const int j = i * 64 + lane;
+                const int l = B::ntx(q) == 1 ? 8 * i + b : 4 * i + (b >> 1);
+                if (j < B::rows(q) && (boundary || l < SZ)) {
+                    const int sl = l < SZ ? l - cz : (cz == 1 ? SZ - 1 : 0);  // source layer, -1 .. SZ
+                    const int zloc = t.wz0 + sl;                             // in the tile: -1 .. tz
+                    const uint32_t zo = (zloc < 0 ? Z[0] : zloc > tzm ? Z[2] : Z[1]) + ((uint32_t)(zloc & tzm) << 6);
+                    const uint32_t xo = cx == 0 ? X[1] : (cx == 1 ? xo_pos : xo_neg);
+                    const uint32_t e = xo + yo[cy + 1] + zo;
+#if defined(__HIP_DEVICE_COMPILE__)  // a gfx950 builtin: absent from the host pass
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rsrc, (__attribute__((address_space(3))) void*)(staged_box + B::offset(q) + i * 1024), 16,
+                        (int)(e << 1), (int)((uint32_t)q * t.tc * 2u), 0, 0);
+#endif
+                }
+            }
+        }
+    } else
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int cx = L::c(q, 0), cy = L::c(q, 1), cz = L::c(q, 2);
@@ -1008,7 +1059,7 @@ __device__ __forceinline__ void step_tile(const StepParams<typename S::R>& p, ui
     if (boundary) m = p.nbmask[(size_t)t.tile * t.tc + t.loc];
     R fv[Q];
     if constexpr (STAGE > 0 && S::id == LBM_STORE_F16C && MODE == MODE_DB && !BIG && L::D == 3)
-        stage_pull<L, STAGE>(t, p.g, p.f_in, boundary, fv);
+        stage_pull<L, STAGE>(t, p.g, p.f_in, p.f_bytes, boundary, fv);
     else
         load_populations<L, S, MODE, BIG>(t, fin, fv);
     constexpr bool links = LINKS && MODE == MODE_DB;

@@ -1394,13 +1394,18 @@ __global__ void __launch_bounds__(MaxThreads<L::Q>::value)
     }
 }
 
+// A 16-byte store that streams past the caches (output fields are not read again by the kernel
+// that writes them; the population lines a neighbouring column still needs stay in L2).
+typedef float lbm_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt4(float* p, lbm_f4 v) { __builtin_nontemporal_store(v, (lbm_f4*)p); }
+
 // calculate_rho_and_velocity (src/lbm/output/velocity.mojo:25-125) in columns: one 512-lane
 // workgroup per column of four z-adjacent 8 x 8 x 8 tiles (Tz a multiple of 4), each lane one node
 // of each tile in turn, at the same place in the tile. A wave reads an 8 x 8 layer of a tile per
 // population load (whole 256-byte runs), and the column's rho and u wait in LDS until all four
 // tiles are done, then leave row-major (z fastest) in whole 128-byte lines: 32 values of rho per
-// (x, y), 96 of u, one 16-byte store per lane and value group. A tile alone would leave 32-byte
-// pieces whose lines three other tiles, 1,024 tiles away in the walk, complete. Boundary tiles
+// (x, y), 96 of u, one 16-byte streaming store per lane and value group. A tile alone would leave
+// 32-byte pieces whose lines three other tiles, 1,024 tiles away in the walk, complete. Boundary tiles
 // (by the tile kinds, uniform per tile) read their nodes' neighbour masks and flags; every wave
 // takes the same four tiles, so a boundary tile delays all of the workgroup's waves alike. The
 // masks are the step's, built from the same flags and bc by the same call, so the moving-wall term
@@ -1474,14 +1479,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     const size_t z0 = (size_t)bz4 * 32;
     auto row = [&](int xy) { return ((size_t)(x0 + (xy & 7)) * ny + (size_t)(y0 + (xy >> 3))) * nz + z0; };
     auto gather4 = [&](const R* lds, int xy, int z) {
-        return make_float4(lds[z * ZS + xy], lds[(z + 1) * ZS + xy], lds[(z + 2) * ZS + xy], lds[(z + 3) * ZS + xy]);
+        return lbm_f4{lds[z * ZS + xy], lds[(z + 1) * ZS + xy], lds[(z + 2) * ZS + xy], lds[(z + 3) * ZS + xy]};
     };
     const int xy = tid >> 3, z4 = (tid & 7) * 4;
     const size_t r = row(xy) + (size_t)z4;
-    *reinterpret_cast<float4*>(p.rho + r) = gather4(srho, xy, z4);
+    store_nt4(p.rho + r, gather4(srho, xy, z4));
     if (p.u_layout != LBM_U_XYZD) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) *reinterpret_cast<float4*>(p.u + (size_t)d * ncell + r) = gather4(su[d], xy, z4);
+        for (int d = 0; d < D; ++d) store_nt4(p.u + (size_t)d * ncell + r, gather4(su[d], xy, z4));
     } else {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -1492,7 +1497,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
                 const int idx = e * 4 + j, z = idx / 3;
                 w[j] = su[idx - 3 * z][z * ZS + gxy];
             }
-            *reinterpret_cast<float4*>(p.u + row(gxy) * 3 + (size_t)e * 4) = make_float4(w[0], w[1], w[2], w[3]);
+            store_nt4(p.u + row(gxy) * 3 + (size_t)e * 4, lbm_f4{w[0], w[1], w[2], w[3]});
         }
     }
 }
@@ -1503,7 +1508,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
 // column and its one-node halo (10 x 10 x 34 nodes, 1.7 velocities read per node against 2.0 for a
 // tile's 10 x 10 x 10 box) is staged in LDS first, with the flags of the same cells when a tile of
 // the column is a boundary tile; Q then waits in LDS and leaves row-major in whole 128-byte lines,
-// one 16-byte store per lane. Boundary tiles take their masks and own flags as macro_fields_col
+// one 16-byte streaming store per lane. Boundary tiles take their masks and own flags as macro_fields_col
 // does; the gradient of a boundary tile reads the flag box (SOLID neighbours at half distance),
 // that of an interior tile (halo all FLUID) none. Same arithmetic as macro_kernel (q_value).
 template <class L, class S, int MODE, bool DDF, bool BIG>
@@ -1615,8 +1620,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
     __syncthreads();
     const int xy = tid >> 3, z4 = (tid & 7) * 4;
     const size_t r = ((size_t)(bx * 8 + (xy & 7)) * ny + (size_t)(by * 8 + (xy >> 3))) * nz + (size_t)(bz4 * 32 + z4);
-    *reinterpret_cast<float4*>(p.qout + r) =
-        make_float4(sq[z4 * ZS + xy], sq[(z4 + 1) * ZS + xy], sq[(z4 + 2) * ZS + xy], sq[(z4 + 3) * ZS + xy]);
+    store_nt4(p.qout + r, lbm_f4{sq[z4 * ZS + xy], sq[(z4 + 1) * ZS + xy], sq[(z4 + 2) * ZS + xy], sq[(z4 + 3) * ZS + xy]});
 }
 
 // Sums the per-tile partials in tile order (one workgroup, lane-strided, then lane order).

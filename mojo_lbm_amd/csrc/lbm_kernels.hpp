@@ -1183,13 +1183,14 @@ __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3],
 }
 
 // Q = 0.25 |omega|^2 - 0.5 |S|^2 of a node (calculate_Q_criterion,
-// src/lbm/output/Q_criterion.mojo:90-215): omega from finite differences of the velocity box su
-// (D values per cell, strides ls, the node at center; FLAGS: SOLID neighbours in sfl, same cell
-// order, at half distance), S from the node's re-streamed populations fv.
-template <class L, class R, bool DDF, bool FLAGS>
-__device__ __forceinline__ R q_value(const int (&pos)[3], const int (&nn)[3], const R* su, const uint8_t* sfl,
-                                     int center, const int (&ls)[3], const R (&fv)[L::Q], R tau) {
-    constexpr int Q = L::Q, D = L::D;
+// src/lbm/output/Q_criterion.mojo:90-215): |omega|^2 from finite differences of the velocity box
+// su (vorticity_sq: D values per cell, strides ls, the node at center; FLAGS: SOLID neighbours in
+// sfl, same cell order, at half distance), S from the node's re-streamed populations fv (q_from);
+// q_value does both.
+template <class L, class R, bool FLAGS>
+__device__ __forceinline__ R vorticity_sq(const int (&pos)[3], const int (&nn)[3], const R* su, const uint8_t* sfl,
+                                          int center, const int (&ls)[3]) {
+    constexpr int D = L::D;
     auto grad = [&](int comp, int axis) { return fd_gradient<R, D, FLAGS>(pos, nn, su, sfl, center, ls, comp, axis); };
     R wsq;
     if (D == 2) {
@@ -1203,6 +1204,13 @@ __device__ __forceinline__ R q_value(const int (&pos)[3], const int (&nn)[3], co
         wsq = wsq + wy * wy;
         wsq = wsq + wz * wz;
     }
+    return wsq;
+}
+
+// Q of a node from |omega|^2 (vorticity_sq) and its re-streamed populations.
+template <class L, class R, bool DDF>
+__device__ __forceinline__ R q_from(R wsq, const R (&fv)[L::Q], R tau) {
+    constexpr int Q = L::Q;
     const R rho = density<L, R, DDF>(fv);
     R u[3];
     velocity<L, R>(fv, rho, u);
@@ -1212,6 +1220,12 @@ __device__ __forceinline__ R q_value(const int (&pos)[3], const int (&nn)[3], co
     for (int q = 0; q < Q; ++q) fn[q] = fv[q] - feq<L, R, DDF>(q, rho, u, uu);
     pi_neq<L, R>(fn, pi);
     return (R)0.25 * wsq - (R)0.5 * strain_norm_sq<L, R>(pi, rho, tau);
+}
+
+template <class L, class R, bool DDF, bool FLAGS>
+__device__ __forceinline__ R q_value(const int (&pos)[3], const int (&nn)[3], const R* su, const uint8_t* sfl,
+                                     int center, const int (&ls)[3], const R (&fv)[L::Q], R tau) {
+    return q_from<L, R, DDF>(vorticity_sq<L, R, FLAGS>(pos, nn, su, sfl, center, ls), fv, tau);
 }
 
 // Loads the velocities of a tile and its one-node halo (cells outside the grid: 0, never
@@ -1510,9 +1524,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
 // the column is a boundary tile; Q then waits in LDS and leaves row-major in whole 128-byte lines,
 // one 16-byte streaming store per lane. Boundary tiles take their masks and own flags as macro_fields_col
 // does; the gradient of a boundary tile reads the flag box (SOLID neighbours at half distance),
-// that of an interior tile (halo all FLUID) none. Same arithmetic as macro_kernel (q_value).
+// that of an interior tile (halo all FLUID) none. Same arithmetic as macro_kernel (q_value). Six
+// waves per SIMD (three workgroups per CU, as many as the 52.5 KB of LDS allow) fit in 80 VGPRs
+// because |omega|^2 is done before the population loads are issued.
 template <class L, class S, int MODE, bool DDF, bool BIG>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8)))
     macro_qcrit_col(MacroParams<typename S::R> p) {
     using R = typename S::R;
     constexpr int Q = L::Q, D = L::D;
@@ -1587,6 +1603,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
         TileCtx<L, Idx<BIG>> t;
         t.init_pos(p.g, t0 + (uint32_t)k * tstride, 0, p.g.tzb, tid);
         const bool boundary = ((kinds >> k) & 1u) != 0u;  // uniform
+        // |omega|^2 from the LDS box first, so that its registers are free again before the
+        // population loads are in flight (unused at SOLID nodes)
+        const int pos[3] = {t.x, t.y, t.z};
+        const int center = ((k * 8 + lz + 1) * HY + (tid >> 3 & 7) + 1) * HX + (tid & 7) + 1;
+        const R wsq = boundary ? vorticity_sq<L, R, true>(pos, nn, su, sfl, center, ls)
+                               : vorticity_sq<L, R, false>(pos, nn, su, sfl, center, ls);
         uint32_t m = 0;
         uint8_t own = LBM_FLAG_FLUID;
         if (boundary) {
@@ -1608,13 +1630,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
             }
             apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
         }
-        R qv = 0;
-        if (own != LBM_FLAG_SOLID) {
-            const int pos[3] = {t.x, t.y, t.z};
-            const int center = ((k * 8 + lz + 1) * HY + (tid >> 3 & 7) + 1) * HX + (tid & 7) + 1;
-            qv = boundary ? q_value<L, R, DDF, true>(pos, nn, su, sfl, center, ls, fv, p.tau)
-                          : q_value<L, R, DDF, false>(pos, nn, su, sfl, center, ls, fv, p.tau);
-        }
+        const R qv = own != LBM_FLAG_SOLID ? q_from<L, R, DDF>(wsq, fv, p.tau) : (R)0;
         sq[(k * 8 + lz) * ZS + lxy] = qv;
     }
     __syncthreads();

@@ -109,13 +109,15 @@ class DeviceRun:
         t.cuda.synchronize()
         return rho.cpu().numpy(), u.cpu().numpy()
 
-    def q_criterion(self, u, tau, after_odd=False, u_layout=0):
-        """Q-criterion of the current state from the velocity field u (host array in u_layout)."""
+    def q_criterion(self, u, tau, after_odd=False, u_layout=0, misalign=False):
+        """Q-criterion of the current state from the velocity field u (host array in u_layout);
+        misalign: the output one element past a 16-byte boundary."""
         t = self.torch
         dt = t.float64 if self.desc.storage == 2 else t.float32
         n = self.desc.nx * self.desc.ny * self.desc.nz
         ud = t.from_numpy(np.ascontiguousarray(u)).to(device="cuda", dtype=dt)
-        q = t.full((n,), float("nan"), dtype=dt, device="cuda")
+        k = 1 if misalign else 0
+        q = t.full((n + k,), float("nan"), dtype=dt, device="cuda")[k:]
         _lib.check(self.L.lbm_q_criterion(self.h.ptr, q.data_ptr(), ud.data_ptr(), self.f.data_ptr(),
                                           self.bc.data_ptr(), self.flags.data_ptr(), float(tau), int(after_odd),
                                           int(u_layout), self.stream))

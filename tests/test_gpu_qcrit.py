@@ -137,9 +137,9 @@ COLUMN_CASES = [("D3Q19", 0, 0, 0, (32, 32, 32), False), ("D3Q19", 1, 1, 1, (32,
 @pytest.mark.parametrize("lat,storage,ddf,method,n,walls", COLUMN_CASES)
 @pytest.mark.parametrize("u_layout", [0, 1])
 def test_q_criterion_tall_grids(gpu, lat, storage, ddf, method, n, walls, u_layout):
-    """Q on grids of a multiple of 4 tiles in z (the shape the fields output takes in columns):
-    bit-identical to the oracle in periodic boxes (with the domain-edge one-sided differences) and
-    in the walled box, both u layouts."""
+    """Q on grids of a multiple of 4 tiles in z (the four-tile column kernel; an output off a
+    16-byte boundary takes the tile kernel): bit-identical to the oracle in periodic boxes (with the
+    domain-edge one-sided differences) and in the walled box, both u layouts."""
     print(f"case qcrit tall grid {lat} storage={storage} ddf={ddf} method={method} n={n} walls={walls} "
           f"u_layout={u_layout}", flush=True)
     desc = U.make_desc(lat, n, storage=storage, ddf=ddf, method=method)
@@ -150,7 +150,8 @@ def test_q_criterion_tall_grids(gpu, lat, storage, ddf, method, n, walls, u_layo
     _steps(run, desc, tau, k)
     after_odd = (k % 2 == 1)
     _, u = run.macroscopic(after_odd=after_odd, u_layout=u_layout)
-    q = run.q_criterion(u, tau, after_odd=after_odd, u_layout=u_layout)
     oq = O.q_criterion(desc, u, run.f_host(), bc, flags, tau, after_odd=after_odd, u_layout=u_layout)
-    assert np.array_equal(q.view(np.uint8), oq.view(np.uint8)), float(np.max(np.abs(q - oq)))
+    for misalign in (False, True):
+        q = run.q_criterion(u, tau, after_odd=after_odd, u_layout=u_layout, misalign=misalign)
+        assert np.array_equal(q.view(np.uint8), oq.view(np.uint8)), (misalign, float(np.max(np.abs(q - oq))))
     assert np.max(np.abs(q)) > 0

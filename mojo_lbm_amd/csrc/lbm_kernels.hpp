@@ -1149,9 +1149,26 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
 // right_dx), right_weight = 1 - left_weight. Neighbours outside the grid are skipped (no
 // periodic wrap), which leaves a single valid side with weight 0. Velocities come from the
 // LDS velocity tile su (D values per halo cell, the flag tile's cell order).
-template <class R, int D, bool FLAGS = true>
+// Direction q of a unit move along +axis (sign 1) or -axis (sign -1).
+template <class L> constexpr int face_q(int axis, int sign) {
+    for (int q = 1; q < L::Q; ++q) {
+        bool match = true;
+        for (int d = 0; d < L::D; ++d) match = match && L::c(q, d) == (d == axis ? sign : 0);
+        if (match) return q;
+    }
+    return -1;
+}
+
+// Where fd_gradient learns that a neighbour is SOLID (raw flag): nowhere (an interior tile, whose
+// halo is all FLUID), the LDS flag tile sfl, or the node's neighbour mask (bit q: the pull
+// neighbour x - c_q is SOLID; its periodic wrap at the domain edge is never consulted, since the
+// gradient takes no neighbour outside the grid).
+enum SolidSource { SOLID_NONE = 0, SOLID_LDS = 1, SOLID_MASK = 2 };
+
+template <class L, class R, int SRC>
 __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3], const R* su, const uint8_t* sfl,
-                                         int center, const int (&lstride)[3], int comp, int axis) {
+                                         uint32_t mask, int center, const int (&lstride)[3], int comp, int axis) {
+    constexpr int D = L::D;
     // A SOLID neighbour sits at half distance (dx 0.5, so the difference is scaled by 1 / dx = 2,
     // exactly); the left weight is 1 - dx_l / (dx_l + dx_r). The quotients take four values,
     // written as constants that round as the division does, so no division is executed.
@@ -1164,13 +1181,15 @@ __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3],
     const bool left_ok = pos[axis] - 1 >= 0;
     if (left_ok) {
         const int c = center - lstride[axis];
-        left_half = FLAGS && sfl[c] == LBM_FLAG_SOLID;
+        left_half = SRC == SOLID_LDS ? sfl[c] == LBM_FLAG_SOLID
+                    : SRC == SOLID_MASK ? ((mask >> face_q<L>(axis, 1)) & 1u) != 0u : false;
         left_grad = (uc - su[c * D + comp]) * (left_half ? (R)2 : (R)1);
     }
     const bool right_ok = pos[axis] + 1 < n[axis];
     if (right_ok) {
         const int c = center + lstride[axis];
-        right_half = FLAGS && sfl[c] == LBM_FLAG_SOLID;
+        right_half = SRC == SOLID_LDS ? sfl[c] == LBM_FLAG_SOLID
+                     : SRC == SOLID_MASK ? ((mask >> face_q<L>(axis, -1)) & 1u) != 0u : false;
         right_grad = (su[c * D + comp] - uc) * (right_half ? (R)2 : (R)1);
     }
     // one side only: dx_l / dx_l = 1 (left) or 0 / dx_r = 0 (right)
@@ -1187,11 +1206,13 @@ __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3],
 // su (vorticity_sq: D values per cell, strides ls, the node at center; FLAGS: SOLID neighbours in
 // sfl, same cell order, at half distance), S from the node's re-streamed populations fv (q_from);
 // q_value does both.
-template <class L, class R, bool FLAGS>
+template <class L, class R, int SRC>
 __device__ __forceinline__ R vorticity_sq(const int (&pos)[3], const int (&nn)[3], const R* su, const uint8_t* sfl,
-                                          int center, const int (&ls)[3]) {
+                                          uint32_t mask, int center, const int (&ls)[3]) {
     constexpr int D = L::D;
-    auto grad = [&](int comp, int axis) { return fd_gradient<R, D, FLAGS>(pos, nn, su, sfl, center, ls, comp, axis); };
+    auto grad = [&](int comp, int axis) {
+        return fd_gradient<L, R, SRC>(pos, nn, su, sfl, mask, center, ls, comp, axis);
+    };
     R wsq;
     if (D == 2) {
         const R w = grad(1, 0) - grad(0, 1);
@@ -1225,7 +1246,8 @@ __device__ __forceinline__ R q_from(R wsq, const R (&fv)[L::Q], R tau) {
 template <class L, class R, bool DDF, bool FLAGS>
 __device__ __forceinline__ R q_value(const int (&pos)[3], const int (&nn)[3], const R* su, const uint8_t* sfl,
                                      int center, const int (&ls)[3], const R (&fv)[L::Q], R tau) {
-    return q_from<L, R, DDF>(vorticity_sq<L, R, FLAGS>(pos, nn, su, sfl, center, ls), fv, tau);
+    return q_from<L, R, DDF>(vorticity_sq<L, R, FLAGS ? SOLID_LDS : SOLID_NONE>(pos, nn, su, sfl, 0u, center, ls), fv,
+                             tau);
 }
 
 // Loads the velocities of a tile and its one-node halo (cells outside the grid: 0, never
@@ -1535,9 +1557,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     static_assert(D == 3 && sizeof(R) == 4, "columns of 3-D tiles, 4-byte moments");
     constexpr int HX = 10, HY = 10, HZ = 34, HC = HX * HY * HZ;  // velocity box: tile column + halo
     constexpr int ZS = 65;
+    static_assert(32 * ZS <= HC * D, "Q staged in the velocity box's LDS");
     __shared__ R su[HC * D];
-    __shared__ uint8_t sfl[HC];
-    __shared__ R sq[32 * ZS];
     const uint32_t n_units = (uint32_t)p.g.n_tiles >> 2;  // columns
     if (blockIdx.x >= n_units) return;                     // a launch of more workgroups than columns
     uint32_t c = blockIdx.x;
@@ -1549,7 +1570,6 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     uint32_t kinds = 0;  // bit k: tile k of the column is a boundary tile (uniform)
 #pragma unroll
     for (int k = 0; k < 4; ++k) kinds |= (p.tile_kind[t0 + (uint32_t)k * tstride] & 1u) << k;
-    const bool any_boundary = kinds != 0u;
     const int nx = p.g.nx, ny = p.g.ny, nz = p.g.nz;
     const size_t ncell = (size_t)nx * ny * nz;
     const bool planar = p.u_layout != LBM_U_XYZD;
@@ -1582,33 +1602,16 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
         for (int s2 = 0; s2 < 2; ++s2)
             if (vok[s2]) su[vlds[s2] + rl] = p.vel[base + voff[s2]];
     }
-    if (any_boundary) {  // flags (tiled layout) of the same cells, lane iz of each row
-        const bool zok = lane < HZ && gz0 + lane >= 0 && gz0 + lane < nz;
-        const int gz = gz0 + lane;
-        const size_t zoff = zok ? (size_t)(gz >> 3) * (size_t)(p.g.Tx * p.g.Ty) * 512u + (size_t)(64 * (gz & 7)) : 0;
-        for (int row = wave; row < HX * HY; row += 8) {
-            const int ix = row / HY, iy = row - ix * HY;
-            const int gx = gx0 + ix, gy = gy0 + iy;
-            if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-            const size_t xyoff = (size_t)((gx >> 3) + p.g.Tx * (gy >> 3)) * 512u + (size_t)((gx & 7) + 8 * (gy & 7));
-            if (zok) sfl[(lane * HY + iy) * HX + ix] = p.flags[xyoff + zoff];
-        }
-    }
     __syncthreads();
     const Field<S, BIG> f(p.f, p.f_bytes);
     const int nn[3] = {nx, ny, nz};
     const int ls[3] = {1, HX, HX * HY};
+    R qk[4] = {0, 0, 0, 0};  // this lane's Q in tiles 0 .. 3 (registers: selected, not indexed)
 #pragma unroll 1
     for (int k = 0; k < 4; ++k) {
         TileCtx<L, Idx<BIG>> t;
         t.init_pos(p.g, t0 + (uint32_t)k * tstride, 0, p.g.tzb, tid);
         const bool boundary = ((kinds >> k) & 1u) != 0u;  // uniform
-        // |omega|^2 from the LDS box first, so that its registers are free again before the
-        // population loads are in flight (unused at SOLID nodes)
-        const int pos[3] = {t.x, t.y, t.z};
-        const int center = ((k * 8 + lz + 1) * HY + (tid >> 3 & 7) + 1) * HX + (tid & 7) + 1;
-        const R wsq = boundary ? vorticity_sq<L, R, true>(pos, nn, su, sfl, center, ls)
-                               : vorticity_sq<L, R, false>(pos, nn, su, sfl, center, ls);
         uint32_t m = 0;
         uint8_t own = LBM_FLAG_FLUID;
         if (boundary) {
@@ -1616,6 +1619,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
             m = p.nbmask[node];
             own = p.flags[node];
         }
+        // |omega|^2 from the LDS box first, so that its registers are free again before the
+        // population loads are in flight (unused at SOLID nodes); SOLID face neighbours of a
+        // boundary tile's node from its mask
+        const int pos[3] = {t.x, t.y, t.z};
+        const int center = ((k * 8 + lz + 1) * HY + (tid >> 3 & 7) + 1) * HX + (tid & 7) + 1;
+        const R wsq = boundary ? vorticity_sq<L, R, SOLID_MASK>(pos, nn, su, nullptr, m, center, ls)
+                               : vorticity_sq<L, R, SOLID_NONE>(pos, nn, su, nullptr, 0u, center, ls);
         R fv[Q];
         load_populations<L, S, MODE, BIG>(t, f, fv);
         if (m) {
@@ -1631,8 +1641,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
             apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
         }
         const R qv = own != LBM_FLAG_SOLID ? q_from<L, R, DDF>(wsq, fv, p.tau) : (R)0;
-        sq[(k * 8 + lz) * ZS + lxy] = qv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) qk[i] = i == k ? qv : qk[i];
     }
+    __syncthreads();  // the box is no longer read: Q is staged in it
+    R* sq = su;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sq[(k * 8 + lz) * ZS + lxy] = qk[k];
     __syncthreads();
     const int xy = tid >> 3, z4 = (tid & 7) * 4;
     const size_t r = ((size_t)(bx * 8 + (xy & 7)) * ny + (size_t)(by * 8 + (xy >> 3))) * nz + (size_t)(bz4 * 32 + z4);

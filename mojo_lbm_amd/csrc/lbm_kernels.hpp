@@ -1143,12 +1143,6 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))
     step_tile<L, S, MODE, COLL, LES_MLP, DDF, false, false>(p, unit >> s, unit, unit & ((1u << s) - 1u), p.g.wzb);
 }
 
-// get_velocity_gradient (src/lbm/kernels/utils/finite_difference.mojo:15-136), dx = 1: the
-// one-sided differences to the -1 / +1 neighbours along `axis`, a SOLID neighbour (raw flag,
-// from the LDS flag tile) at half distance, blended by left_weight = 1 - left_dx / (left_dx +
-// right_dx), right_weight = 1 - left_weight. Neighbours outside the grid are skipped (no
-// periodic wrap), which leaves a single valid side with weight 0. Velocities come from the
-// LDS velocity tile su (D values per halo cell, the flag tile's cell order).
 // Direction q of a unit move along +axis (sign 1) or -axis (sign -1).
 template <class L> constexpr int face_q(int axis, int sign) {
     for (int q = 1; q < L::Q; ++q) {
@@ -1165,6 +1159,12 @@ template <class L> constexpr int face_q(int axis, int sign) {
 // gradient takes no neighbour outside the grid).
 enum SolidSource { SOLID_NONE = 0, SOLID_LDS = 1, SOLID_MASK = 2 };
 
+// get_velocity_gradient (src/lbm/kernels/utils/finite_difference.mojo:15-136), dx = 1: the
+// one-sided differences to the -1 / +1 neighbours along `axis`, a SOLID neighbour (raw flag, from
+// the LDS flag tile or the node's mask, by SRC) at half distance, blended by left_weight =
+// 1 - left_dx / (left_dx + right_dx), right_weight = 1 - left_weight. Neighbours outside the grid
+// are skipped (no periodic wrap), which leaves a single valid side with weight 0. Velocities come
+// from the LDS velocity box su (D values per cell, strides lstride, the node at center).
 template <class L, class R, int SRC>
 __device__ __forceinline__ R fd_gradient(const int (&pos)[3], const int (&n)[3], const R* su, const uint8_t* sfl,
                                          uint32_t mask, int center, const int (&lstride)[3], int comp, int axis) {
@@ -1542,13 +1542,13 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
 // macro_fields_col: one 512-lane workgroup per four z-adjacent 8 x 8 x 8 tiles, each lane one node
 // of each tile in turn (a wave reads an 8 x 8 layer per population load). The velocity box of the
 // column and its one-node halo (10 x 10 x 34 nodes, 1.7 velocities read per node against 2.0 for a
-// tile's 10 x 10 x 10 box) is staged in LDS first, with the flags of the same cells when a tile of
-// the column is a boundary tile; Q then waits in LDS and leaves row-major in whole 128-byte lines,
-// one 16-byte streaming store per lane. Boundary tiles take their masks and own flags as macro_fields_col
-// does; the gradient of a boundary tile reads the flag box (SOLID neighbours at half distance),
-// that of an interior tile (halo all FLUID) none. Same arithmetic as macro_kernel (q_value). Six
-// waves per SIMD (three workgroups per CU, as many as the 52.5 KB of LDS allow) fit in 80 VGPRs
-// because |omega|^2 is done before the population loads are issued.
+// tile's 10 x 10 x 10 box) is staged in LDS first; Q then waits in LDS and leaves row-major in whole
+// 128-byte lines, one 16-byte streaming store per lane. Boundary tiles take their masks and own
+// flags as macro_fields_col does, and the gradient of a boundary tile's node finds its SOLID face
+// neighbours (at half distance) in the node's mask; that of an interior tile (halo all FLUID)
+// needs none. Same arithmetic as macro_kernel (q_value). Six waves per SIMD (three workgroups per
+// CU; 49 KB of LDS) fit in 80 VGPRs because |omega|^2 is done before the population loads are
+// issued.
 template <class L, class S, int MODE, bool DDF, bool BIG>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8)))
     macro_qcrit_col(MacroParams<typename S::R> p) {
@@ -1557,8 +1557,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     static_assert(D == 3 && sizeof(R) == 4, "columns of 3-D tiles, 4-byte moments");
     constexpr int HX = 10, HY = 10, HZ = 34, HC = HX * HY * HZ;  // velocity box: tile column + halo
     constexpr int ZS = 65;
-    static_assert(32 * ZS <= HC * D, "Q staged in the velocity box's LDS");
     __shared__ R su[HC * D];
+    __shared__ R sq[32 * ZS];
     const uint32_t n_units = (uint32_t)p.g.n_tiles >> 2;  // columns
     if (blockIdx.x >= n_units) return;                     // a launch of more workgroups than columns
     uint32_t c = blockIdx.x;
@@ -1606,7 +1606,6 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     const Field<S, BIG> f(p.f, p.f_bytes);
     const int nn[3] = {nx, ny, nz};
     const int ls[3] = {1, HX, HX * HY};
-    R qk[4] = {0, 0, 0, 0};  // this lane's Q in tiles 0 .. 3 (registers: selected, not indexed)
 #pragma unroll 1
     for (int k = 0; k < 4; ++k) {
         TileCtx<L, Idx<BIG>> t;
@@ -1640,14 +1639,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
             }
             apply_boundaries<L, S, MODE, DDF, BIG>(t, p.g, m, p.flags, f, p.bc, p.moving, p.eqbc, fv, rebuilt);
         }
-        const R qv = own != LBM_FLAG_SOLID ? q_from<L, R, DDF>(wsq, fv, p.tau) : (R)0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) qk[i] = i == k ? qv : qk[i];
+        sq[(k * 8 + lz) * ZS + lxy] = own != LBM_FLAG_SOLID ? q_from<L, R, DDF>(wsq, fv, p.tau) : (R)0;
     }
-    __syncthreads();  // the box is no longer read: Q is staged in it
-    R* sq = su;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sq[(k * 8 + lz) * ZS + lxy] = qk[k];
     __syncthreads();
     const int xy = tid >> 3, z4 = (tid & 7) * 4;
     const size_t r = ((size_t)(bx * 8 + (xy & 7)) * ny + (size_t)(by * 8 + (xy >> 3))) * nz + (size_t)(bz4 * 32 + z4);

@@ -142,7 +142,9 @@ int lbm_step_ep(lbm_handle* h, void* f, const void* bc, const uint8_t* flags,
  * rho (nx*ny*nz, row-major (x,y,z)) and u (row-major (x,y,z,d) or (d,x,y,z))
  * after re-streaming and re-applying the BCs: replaces calculate_rho_and_velocity
  * (src/lbm/output/velocity.mojo:25-125). after_odd selects the esoteric load
- * pattern (ignored for double buffer).
+ * pattern (ignored for double buffer). Results do not depend on alignment; 16-byte aligned rho
+ * and u (with 8x8x8 tiles, a multiple of 4 tiles in z, FP32 or Float16C) take the faster
+ * whole-line kernel.
  */
 int lbm_macroscopic(lbm_handle* h, void* rho, void* u, const void* f, const void* bc,
                     const uint8_t* flags, int after_odd, int u_layout, void* stream);
@@ -156,7 +158,8 @@ int lbm_macroscopic(lbm_handle* h, void* rho, void* u, const void* f, const void
  * a SOLID neighbour sits at half distance, a neighbour outside the grid is skipped and a
  * derivative with one valid side is 0. S = Pi_neq * (-1 / (2 rho cs^2 tau)) from the
  * re-streamed, boundary-applied populations (after_odd as for lbm_macroscopic). SOLID nodes
- * (raw flag) get 0; the reference leaves them unwritten.
+ * (raw flag) get 0; the reference leaves them unwritten. A 16-byte aligned q takes the faster
+ * whole-line kernel under the same conditions as lbm_macroscopic.
  */
 int lbm_q_criterion(lbm_handle* h, void* q, const void* u, const void* f, const void* bc,
                     const uint8_t* flags, double tau, int after_odd, int u_layout, void* stream);

@@ -1538,12 +1538,17 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     }
 }
 
+// Dynamic LDS of macro_qcrit_col: the velocity box (cells x D values) and the Q staging buffer,
+// sizeof(R) bytes each.
+constexpr int kQColBoxCells = 10 * 10 * 34, kQColStage = 32 * 65;
+
 // calculate_Q_criterion (src/lbm/output/Q_criterion.mojo:90-215) in the columns of
 // macro_fields_col: one 512-lane workgroup per four z-adjacent 8 x 8 x 8 tiles, each lane one node
 // of each tile in turn (a wave reads an 8 x 8 layer per population load). The velocity box of the
 // column and its one-node halo (10 x 10 x 34 nodes, 1.7 velocities read per node against 2.0 for a
-// tile's 10 x 10 x 10 box) is staged in LDS first; Q then waits in LDS and leaves row-major in whole
-// 128-byte lines, one 16-byte streaming store per lane. Boundary tiles take their masks and own
+// tile's 10 x 10 x 10 box) is staged in LDS first (interleaved u: straight from memory to LDS,
+// without registers); Q then waits in LDS and leaves row-major in whole 128-byte lines, one 16-byte
+// streaming store per lane. Boundary tiles take their masks and own
 // flags as macro_fields_col does, and the gradient of a boundary tile's node finds its SOLID face
 // neighbours (at half distance) in the node's mask; that of an interior tile (halo all FLUID)
 // needs none. Same arithmetic as macro_kernel (q_value). Six waves per SIMD (three workgroups per
@@ -1557,8 +1562,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     static_assert(D == 3 && sizeof(R) == 4, "columns of 3-D tiles, 4-byte moments");
     constexpr int HX = 10, HY = 10, HZ = 34, HC = HX * HY * HZ;  // velocity box: tile column + halo
     constexpr int ZS = 65;
-    __shared__ R su[HC * D];
-    __shared__ R sq[32 * ZS];
+    static_assert(HC == kQColBoxCells && 32 * ZS == kQColStage, "LDS size the launch gives");
+    R* su = (R*)staged_box;        // HC * D: cells (x, y) rows of 34 z, D values each
+    R* sq = su + HC * D;           // 32 * ZS
     const uint32_t n_units = (uint32_t)p.g.n_tiles >> 2;  // columns
     if (blockIdx.x >= n_units) return;                     // a launch of more workgroups than columns
     uint32_t c = blockIdx.x;
@@ -1574,11 +1580,11 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
     const size_t ncell = (size_t)nx * ny * nz;
     const bool planar = p.u_layout != LBM_U_XYZD;
     const int gx0 = bx * 8 - 1, gy0 = by * 8 - 1, gz0 = bz4 * 32 - 1;
-    // The boxes row by row: wave w takes rows (x, y) w, w + 8, ... of the 10 x 10; a row is the 34 z
-    // of the box, D values each (102 consecutive words in the interleaved layout, D runs of 34 in
-    // the planar one), lane j and j + 64 reading word j and j + 64 of it. The per-lane part of the
-    // addresses is fixed; the row adds a uniform offset. Cells outside the grid are never read by
-    // the gradient and are left as they are.
+    // The box row by row, cells (x, y, z) z fastest: wave w takes rows (x, y) w, w + 8, ... of the
+    // 10 x 10; a row is the 34 z of the box, D values each. Interleaved layout: the row's 102
+    // consecutive words go straight from memory to LDS (lane j's word at the row's LDS base + 4 j);
+    // planar: lane j and j + 64 carry word j and j + 64 of the row through registers. Cells outside
+    // the grid are never read by the gradient and are left as they are.
     const int lane = tid & 63, wave = tid >> 6;
     int64_t voff[2];
     int vlds[2];
@@ -1589,23 +1595,33 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
         const int iz = planar ? j % HZ : j / D, d = planar ? j / HZ : j % D;
         vok[s2] = j < HZ * D && gz0 + iz >= 0 && gz0 + iz < nz;
         voff[s2] = planar ? (int64_t)d * (int64_t)ncell + iz : (int64_t)j;
-        vlds[s2] = iz * (HX * HY * D) + d;
+        vlds[s2] = iz * D + d;
     }
     for (int row = wave; row < HX * HY; row += 8) {
         const int ix = row / HY, iy = row - ix * HY;  // uniform
         const int gx = gx0 + ix, gy = gy0 + iy;
         if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
         const int64_t rz = ((int64_t)gx * ny + gy) * nz + gz0;  // row-major index of (gx, gy, gz0)
-        const int64_t base = planar ? rz : rz * D;
-        const int rl = (iy * HX + ix) * D;
+        const int rl = row * (HZ * D);
+        if (planar) {
 #pragma unroll
-        for (int s2 = 0; s2 < 2; ++s2)
-            if (vok[s2]) su[vlds[s2] + rl] = p.vel[base + voff[s2]];
+            for (int s2 = 0; s2 < 2; ++s2)
+                if (vok[s2]) su[vlds[s2] + rl] = p.vel[rz + voff[s2]];
+        } else {
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2)
+                if (vok[s2]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+                    __builtin_amdgcn_global_load_lds(p.vel + rz * D + voff[s2], (uint8_t*)(su + rl + 64 * s2), 4, 0, 0);
+#endif
+                }
+        }
     }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const Field<S, BIG> f(p.f, p.f_bytes);
     const int nn[3] = {nx, ny, nz};
-    const int ls[3] = {1, HX, HX * HY};
+    const int ls[3] = {HY * HZ, HZ, 1};
 #pragma unroll 1
     for (int k = 0; k < 4; ++k) {
         TileCtx<L, Idx<BIG>> t;
@@ -1622,7 +1638,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 8))
         // population loads are in flight (unused at SOLID nodes); SOLID face neighbours of a
         // boundary tile's node from its mask
         const int pos[3] = {t.x, t.y, t.z};
-        const int center = ((k * 8 + lz + 1) * HY + (tid >> 3 & 7) + 1) * HX + (tid & 7) + 1;
+        const int center = (((tid & 7) + 1) * HY + (tid >> 3 & 7) + 1) * HZ + k * 8 + lz + 1;
         const R wsq = boundary ? vorticity_sq<L, R, SOLID_MASK>(pos, nn, su, nullptr, m, center, ls)
                                : vorticity_sq<L, R, SOLID_NONE>(pos, nn, su, nullptr, 0u, center, ls);
         R fv[Q];

@@ -171,10 +171,11 @@ template <class L, class S> struct LaunchImpl {
             if (!((uintptr_t)p.qout & 15u) && p.tile_kind && p.nbmask && h.g.txb == 3 && h.g.tyb == 3 &&
                 h.g.tzb == 3 && h.g.Tz % 4 == 0) {
                 const dim3 grid((unsigned)(h.g.n_tiles / 4));
+                const size_t qlds = (size_t)(kQColBoxCells * L::D + kQColStage) * sizeof(R);
                 if (h.big)
-                    hipLaunchKernelGGL((macro_qcrit_col<L, S, MODE, DDF, true>), grid, dim3(512), 0, st, p);
+                    hipLaunchKernelGGL((macro_qcrit_col<L, S, MODE, DDF, true>), grid, dim3(512), qlds, st, p);
                 else
-                    hipLaunchKernelGGL((macro_qcrit_col<L, S, MODE, DDF, false>), grid, dim3(512), 0, st, p);
+                    hipLaunchKernelGGL((macro_qcrit_col<L, S, MODE, DDF, false>), grid, dim3(512), qlds, st, p);
                 return check_launch("macro_qcrit_col");
             }
         }

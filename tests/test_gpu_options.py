@@ -208,22 +208,31 @@ def test_step_lanes_errors(gpu):
     _lib.check(run.L.lbm_set_option(run.h.ptr, _lib.OPT_STEP_LANES, 0))
 
 
-# (lattice, collision, les, ddf, walls, grid, eq): Float16C double buffer on 8 x 8 x 8 tiles in
-# 256-lane slabs, the shape the staged pull serves
-STAGED_CASES = [("D3Q19", 0, 0, 1, True, (32, 16, 24), False), ("D3Q19", 0, 1, 0, True, (16, 32, 16), False),
-                ("D3Q19", 2, 0, 1, False, (24, 16, 32), False), ("D3Q19", 1, 1, 1, True, (16, 16, 16), True),
-                ("D3Q19", 3, 0, 0, True, (16, 24, 8), False),
-                ("D3Q27", 0, 0, 1, True, (16, 16, 24), False), ("D3Q27", 2, 1, 1, False, (24, 16, 16), False)]
+# (lattice, collision, les, ddf, walls, grid, eq, tile): Float16C double buffer on 8 x 8 x tz tiles
+# in 256-lane slabs, the shape the staged pull serves (two slabs per 8 x 8 x 8 tile, one per
+# 8 x 8 x 4 tile)
+STAGED_CASES = [("D3Q19", 0, 0, 1, True, (32, 16, 24), False, (8, 8, 8)),
+                ("D3Q19", 0, 1, 0, True, (16, 32, 16), False, (8, 8, 8)),
+                ("D3Q19", 2, 0, 1, False, (24, 16, 32), False, (8, 8, 8)),
+                ("D3Q19", 1, 1, 1, True, (16, 16, 16), True, (8, 8, 8)),
+                ("D3Q19", 3, 0, 0, True, (16, 24, 8), False, (8, 8, 8)),
+                ("D3Q27", 0, 0, 1, True, (16, 16, 24), False, (8, 8, 8)),
+                ("D3Q27", 2, 1, 1, False, (24, 16, 16), False, (8, 8, 8)),
+                ("D3Q19", 0, 0, 1, True, (16, 16, 12), False, (8, 8, 4)),
+                ("D3Q19", 0, 0, 1, False, (16, 8, 8), False, (8, 8, 4)),
+                ("D3Q27", 1, 0, 1, False, (8, 16, 12), False, (8, 8, 4))]
 
 
-@pytest.mark.parametrize("lat,coll,les,ddf,walls,grid,eq", STAGED_CASES)
-def test_staged_pull_matches_oracle(gpu, lat, coll, les, ddf, walls, grid, eq):
+@pytest.mark.parametrize("lat,coll,les,ddf,walls,grid,eq,tile", STAGED_CASES)
+def test_staged_pull_matches_oracle(gpu, lat, coll, les, ddf, walls, grid, eq, tile):
     """The direct pull (LBM_OPT_STAGED_PULL 0), the staged pull per tile (1) and per x-adjacent tile
     pair (2) against the oracle: populations bit-identical after every step, periodic wrap in x, y
-    and z included (periodic boxes), walls with a moving lid, equilibrium nodes on one face; an odd
-    tile count in x falls back to one tile per workgroup."""
-    print(f"case staged pull {lat} coll={coll} les={les} ddf={ddf} walls={walls} grid={grid} eq={eq}", flush=True)
-    desc = U.make_desc(lat, grid, storage=1, ddf=ddf, collision=coll, les=les, eq=eq)
+    and z included (periodic boxes, also with one tile per axis), walls with a moving lid,
+    equilibrium nodes on one face, tiles 8 and 4 nodes high; an odd tile count in x falls back to
+    one tile per workgroup."""
+    print(f"case staged pull {lat} coll={coll} les={les} ddf={ddf} walls={walls} grid={grid} eq={eq} tile={tile}",
+          flush=True)
+    desc = U.make_desc(lat, grid, storage=1, ddf=ddf, collision=coll, les=les, eq=eq, tile=tile)
     f, bc, flags = U.perturbed_state(desc, seed=31 + coll, walls=walls)
     if eq:  # the +X face an equilibrium outlet (rho given, u free)
         O.set_exterior_wall(desc, flags, bc, "+X", 2, None, 1.0)
